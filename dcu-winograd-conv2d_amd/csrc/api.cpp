@@ -1,0 +1,501 @@
+// api.cpp -- the C ABI: the reference entry points getParamsize /
+// getkernelInfo / conv2d_fp16 (include/conv2d.h:39-42, src/conv2d.cpp:16-133)
+// plus the extended controls of winoconv.h.  Host code only; kernels are in
+// kernels_*.hip.
+#include <stdio.h>
+#include <string.h>
+
+#include "wino_internal.h"
+
+using namespace wino;
+
+namespace wino {
+
+static uint64_t round_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
+static uint64_t align256(uint64_t a) { return round_up(a, 256); }
+
+// The reference selection rule (src/conv2d.cpp:76-114), restated as a pure
+// function: 32-bit unsigned sizes as in the reference, evaluated with the
+// F(4x3) tile sizes every fresh process starts with (src/conv2d.cpp:3-8).
+int reference_select(const problem_t* pr) {
+  const uint32_t outh = (pr->h - pr->r + 2 * pr->p) / pr->u + 1;
+  const uint32_t outw = (pr->w - pr->s + 2 * pr->q) / pr->v + 1;
+  const uint64_t P4 = (uint64_t)pr->n * ((outh + 3ull) / 4) * ((outw + 3ull) / 4);
+  const uint32_t image_size = pr->n * pr->c * pr->h * pr->w;
+  const uint32_t filter_size = pr->k * pr->c * pr->r * pr->s;
+  const uint32_t U_size = (uint32_t)(36ull * pr->k * pr->c);
+  const uint32_t V_size = (uint32_t)(36ull * P4 * pr->c);
+  if (image_size >= 10u * filter_size) return WINO_ALGO_F23_FUSED;
+  if (U_size > 4u * V_size) return WINO_ALGO_F23;
+  return WINO_ALGO_F43;
+}
+
+int validate(const problem_t* pr) {
+  if (!pr) return WINO_ERR_INVALID;
+  if (pr->r != 3 || pr->s != 3 || pr->u != 1 || pr->v != 1 || pr->p != 1 || pr->q != 1)
+    return WINO_ERR_UNSUPPORTED;
+  if (pr->n == 0 || pr->c == 0 || pr->h == 0 || pr->w == 0 || pr->k == 0)
+    return WINO_ERR_UNSUPPORTED;
+  // element counts of every tensor must fit the 32-bit grid decomposition
+  const uint64_t in = (uint64_t)pr->n * pr->c * pr->h * pr->w;
+  const uint64_t out = (uint64_t)pr->n * pr->k * pr->h * pr->w;
+  const uint64_t lim = 1ull << 31;
+  if (in >= lim || out >= lim || (uint64_t)pr->k * pr->c * 9 >= lim) return WINO_ERR_UNSUPPORTED;
+  return WINO_OK;
+}
+
+// GEMM tile choice: least padded work, then enough blocks to fill 256 CUs,
+// then the larger tile (more operand reuse).
+static int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision) {
+  int best = 0;
+  double best_score = -1.0;
+  for (int i = 0; i < gemm_cfg_count(); ++i) {
+    const GemmCfg c = gemm_cfg(i);
+    if (precision != WINO_PREC_FP16 && c.bk * c.bp < 64 * 64) continue;
+    const double kp = (double)round_up(K, c.bk), pp = (double)round_up(P, c.bp);
+    const double eff = (double)K * (double)P / (kp * pp);
+    const double blocks = kp / c.bk * pp / c.bp * t2;
+    const double fill = blocks >= 512.0 ? 1.0 : blocks / 512.0;
+    const double reuse = (double)c.bk * c.bp / (c.bk + c.bp) / 64.0;  // 1.0 for 128x128
+    const double score = eff * fill * (0.75 + 0.25 * reuse);
+    if (score > best_score) {
+      best_score = score;
+      best = i;
+    }
+  }
+  return best;
+}
+
+int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
+  memset(pl, 0, sizeof(*pl));
+  if (algo == WINO_ALGO_AUTO) algo = reference_select(pr);
+  if (algo < WINO_ALGO_F43 || algo > WINO_ALGO_F43_FUSED) return WINO_ERR_INVALID;
+  if (precision < WINO_PREC_FP16 || precision > WINO_PREC_FP8X2) return WINO_ERR_INVALID;
+  if (algo == WINO_ALGO_F43_FUSED) return WINO_ERR_UNSUPPORTED;
+  if (algo == WINO_ALGO_F23_FUSED && precision != WINO_PREC_FP16) return WINO_ERR_UNSUPPORTED;
+  pl->algo = algo;
+  pl->precision = precision;
+  pl->m = (algo == WINO_ALGO_F43 || algo == WINO_ALGO_F43_FUSED) ? 4 : 2;
+  pl->T = pl->m + 2;
+  pl->t2 = pl->T * pl->T;
+  pl->oh = (pr->h + 2 * pr->p - pr->r) / pr->u + 1;
+  pl->ow = (pr->w + 2 * pr->q - pr->s) / pr->v + 1;
+  pl->tiles_h = (pl->oh + pl->m - 1) / pl->m;
+  pl->tiles_w = (pl->ow + pl->m - 1) / pl->m;
+  pl->P = (uint64_t)pr->n * pl->tiles_h * pl->tiles_w;
+  pl->planes = precision == WINO_PREC_FP8X2 ? 2 : 1;
+  if (algo == WINO_ALGO_F23_FUSED) {
+    pl->gemm_cfg = -1;
+    pl->bk = 32;
+    pl->bp = 64;
+    pl->Kp = (uint32_t)round_up(pr->k, 32);
+    pl->Cp = (uint32_t)round_up(pr->c, 32);
+    pl->Pp = pl->P;
+    pl->u_bytes = (uint64_t)pl->t2 * pl->Cp * pl->Kp * 2;
+    pl->u_off = 0;
+    pl->ws_bytes = align256(pl->u_bytes);
+    return WINO_OK;
+  }
+  pl->gemm_cfg = choose_gemm_cfg(pr->k, pl->P, pl->t2, precision);
+  const GemmCfg gc = gemm_cfg(pl->gemm_cfg);
+  pl->bk = gc.bk;
+  pl->bp = gc.bp;
+  pl->Kp = (uint32_t)round_up(pr->k, gc.bk);
+  pl->Pp = round_up(pl->P, gc.bp);
+  if (precision == WINO_PREC_FP16) {
+    pl->Cp = (uint32_t)round_up(pr->c, 32);
+    pl->u_bytes = (uint64_t)pl->t2 * pl->Cp * pl->Kp * 2;
+    pl->v_bytes = (uint64_t)pl->t2 * pl->Cp * pl->Pp * 2;
+  } else {
+    pl->Cp = (uint32_t)round_up(pr->c, 128);
+    pl->u_bytes = (uint64_t)pl->planes * pl->t2 * pl->Cp * pl->Kp;
+    pl->v_bytes = (uint64_t)pl->planes * pl->t2 * pl->Cp * pl->Pp;
+    pl->scale_bytes = (uint64_t)pl->t2 * pl->Kp * 4 + 256 + 256;
+  }
+  pl->m_bytes = (uint64_t)pl->t2 * pl->Kp * pl->Pp * 2;
+  uint64_t off = 0;
+  pl->u_off = off;
+  off += align256(pl->u_bytes);
+  pl->v_off = off;
+  off += align256(pl->v_bytes);
+  pl->m_off = off;
+  off += align256(pl->m_bytes);
+  if (precision != WINO_PREC_FP16) {
+    pl->us_off = off;
+    off += align256((uint64_t)pl->t2 * pl->Kp * 4);
+    pl->vs_off = off;
+    off += 256;
+    pl->amax_off = off;
+    off += 256;
+  }
+  pl->ws_bytes = off;
+  return WINO_OK;
+}
+
+static Param* as_param(const void* p) {
+  Param* prm = (Param*)p;
+  if (!prm || prm->magic != kMagic) return nullptr;
+  return prm;
+}
+
+static int set_status(Param* prm, int st) {
+  prm->status = st;
+  return st;
+}
+
+static int hip_status(hipError_t e) {
+  if (e == hipSuccess) return WINO_OK;
+  fprintf(stderr, "winoconv: HIP error: %s\n", hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? WINO_ERR_ALLOC : WINO_ERR_LAUNCH;
+}
+
+// (Re)allocates and zeroes the workspace for prm->plan.
+static int alloc_workspace(Param* prm) {
+  if (prm->ws) {
+    (void)hipFree(prm->ws);
+    prm->ws = nullptr;
+  }
+  prm->filter_ready = 0;
+  if (prm->plan.ws_bytes == 0) return WINO_OK;
+  void* ws = nullptr;
+  hipError_t e = hipMalloc(&ws, prm->plan.ws_bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return WINO_ERR_ALLOC;
+  }
+  e = hipMemset(ws, 0, prm->plan.ws_bytes);  // pads of U/V must read as zero
+  if (e == hipSuccess) e = hipDeviceSynchronize();  // setup time only
+  if (e != hipSuccess) {
+    (void)hipFree(ws);
+    return hip_status(e);
+  }
+  prm->ws = ws;
+  return WINO_OK;
+}
+
+static char* wsp(const Param* prm, uint64_t off) { return (char*)prm->ws + off; }
+
+static int stage_filter(Param* prm) {
+  const Plan& pl = prm->plan;
+  float* us = pl.precision != WINO_PREC_FP16 ? (float*)wsp(prm, pl.us_off) : nullptr;
+  return hip_status(launch_filter_transform(pl, prm->weight, prm->k, prm->c, wsp(prm, pl.u_off),
+                                            us, prm->stream));
+}
+
+static int stage_input(Param* prm) {
+  const Plan& pl = prm->plan;
+  float* amax = nullptr;
+  float* vs = nullptr;
+  if (pl.precision != WINO_PREC_FP16) {
+    amax = (float*)wsp(prm, pl.amax_off);
+    vs = (float*)wsp(prm, pl.vs_off);
+    int st = hip_status(launch_input_amax(prm->in, (uint64_t)prm->n * prm->c * prm->h * prm->w,
+                                          amax, prm->stream));
+    if (st) return st;
+  }
+  return hip_status(launch_input_transform(pl, prm->in, prm->n, prm->c, prm->h, prm->w,
+                                           wsp(prm, pl.v_off), amax, vs, prm->stream));
+}
+
+static int stage_gemm(Param* prm) {
+  const Plan& pl = prm->plan;
+  const float* us = pl.precision != WINO_PREC_FP16 ? (const float*)wsp(prm, pl.us_off) : nullptr;
+  const float* vs = pl.precision != WINO_PREC_FP16 ? (const float*)wsp(prm, pl.vs_off) : nullptr;
+  return hip_status(launch_gemm(pl, wsp(prm, pl.u_off), wsp(prm, pl.v_off),
+                                (_Float16*)wsp(prm, pl.m_off), us, vs, prm->stream));
+}
+
+static int stage_output(Param* prm) {
+  const Plan& pl = prm->plan;
+  return hip_status(launch_output_transform(pl, (const _Float16*)wsp(prm, pl.m_off), prm->n,
+                                            prm->k, prm->out, prm->stream));
+}
+
+static int run(Param* prm) {
+  const Plan& pl = prm->plan;
+  int st = WINO_OK;
+  if (!(prm->resident && prm->filter_ready)) {
+    st = stage_filter(prm);
+    if (st) return st;
+    prm->filter_ready = 1;
+  }
+  if (pl.algo == WINO_ALGO_F23_FUSED) {
+    return hip_status(launch_fused_f23(pl, prm->in, wsp(prm, pl.u_off), prm->n, prm->c, prm->h,
+                                       prm->w, prm->k, prm->out, prm->stream));
+  }
+  if ((st = stage_input(prm))) return st;
+  if ((st = stage_gemm(prm))) return st;
+  return stage_output(prm);
+}
+
+static int replan(Param* prm, int algo, int precision) {
+  problem_t pr;
+  pr.in = prm->in;
+  pr.weight = prm->weight;
+  pr.out = prm->out;
+  pr.n = prm->n;
+  pr.c = prm->c;
+  pr.h = prm->h;
+  pr.w = prm->w;
+  pr.k = prm->k;
+  pr.r = prm->r;
+  pr.s = prm->s;
+  pr.u = prm->u;
+  pr.v = prm->v;
+  pr.p = prm->p;
+  pr.q = prm->q;
+  Plan pl;
+  int st = make_plan(&pr, algo, precision, &pl);
+  if (st) return st;
+  prm->plan = pl;
+  return alloc_workspace(prm);
+}
+
+}  // namespace wino
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+const char* wino_status_string(int st) {
+  switch (st) {
+    case WINO_OK: return "ok";
+    case WINO_ERR_INVALID: return "invalid argument";
+    case WINO_ERR_UNSUPPORTED: return "unsupported problem";
+    case WINO_ERR_ALLOC: return "device allocation failed";
+    case WINO_ERR_LAUNCH: return "kernel launch failed";
+    case WINO_ERR_CAPACITY: return "capacity exceeded";
+    case WINO_ERR_BUSY: return "not ready";
+    case WINO_ERR_NOT_FOUND: return "not found";
+    default: return "unknown status";
+  }
+}
+
+int wino_select_algo(const problem_t* problem) {
+  if (!problem) return WINO_ERR_INVALID;
+  return reference_select(problem);
+}
+
+int wino_validate(const problem_t* problem) { return validate(problem); }
+
+// src/conv2d.cpp:16-21
+int getParamsize(problem_t* problem, int* paramSize) {
+  (void)problem;
+  if (!paramSize) return WINO_ERR_INVALID;
+  *paramSize = (int)sizeof(Param);
+  return 0;
+}
+
+// src/conv2d.cpp:24-116
+int getkernelInfo(problem_t* problem, kernelInfo_t* kernelInfo, void* param) {
+  if (!problem || !param) return WINO_ERR_INVALID;
+  Param* prm = (Param*)param;
+  memset(prm, 0, sizeof(Param));
+  prm->magic = kMagic;
+  prm->in = problem->in;
+  prm->weight = problem->weight;
+  prm->out = problem->out;
+  prm->n = problem->n;
+  prm->c = problem->c;
+  prm->h = problem->h;
+  prm->w = problem->w;
+  prm->k = problem->k;
+  prm->r = problem->r;
+  prm->s = problem->s;
+  prm->u = problem->u;
+  prm->v = problem->v;
+  prm->p = problem->p;
+  prm->q = problem->q;
+  int st = validate(problem);
+  if (st) return set_status(prm, st);
+  st = replan(prm, WINO_ALGO_AUTO, WINO_PREC_FP16);
+  if (st) return set_status(prm, st);
+  if (kernelInfo) {
+    memset(kernelInfo, 0, sizeof(*kernelInfo));
+    const Plan& pl = prm->plan;
+    if (pl.algo == WINO_ALGO_F23_FUSED) {
+      kernelInfo->blockx = (uint32_t)((pl.P + 63) / 64) * (pl.Kp / 32);
+      kernelInfo->threadx = 256;
+    } else {
+      kernelInfo->blockx = (uint32_t)(pl.Pp / pl.bp);
+      kernelInfo->blocky = pl.Kp / pl.bk;
+      kernelInfo->blockz = pl.t2;
+      kernelInfo->threadx = gemm_cfg(pl.gemm_cfg).threads;
+    }
+    kernelInfo->thready = kernelInfo->threadz = 1;
+    if (!kernelInfo->blocky) kernelInfo->blocky = 1;
+    if (!kernelInfo->blockz) kernelInfo->blockz = 1;
+  }
+  return set_status(prm, WINO_OK);
+}
+
+// src/conv2d.cpp:118-133
+void conv2d_fp16(const void* param_ptr) {
+  Param* prm = as_param(param_ptr);
+  if (!prm) {
+    fprintf(stderr, "winoconv: conv2d_fp16: uninitialised parameter blob\n");
+    return;
+  }
+  if (prm->status != WINO_OK && !prm->ws && prm->plan.ws_bytes) return;
+  int st = run(prm);
+  if (st) fprintf(stderr, "winoconv: conv2d_fp16: %s\n", wino_status_string(st));
+  prm->status = st;
+}
+
+int wino_run(const void* param_ptr) {
+  Param* prm = as_param(param_ptr);
+  if (!prm) return WINO_ERR_INVALID;
+  if (prm->plan.t2 == 0) return set_status(prm, WINO_ERR_UNSUPPORTED);
+  return set_status(prm, run(prm));
+}
+
+int wino_param_status(const void* param) {
+  Param* prm = as_param(param);
+  return prm ? prm->status : WINO_ERR_INVALID;
+}
+
+int wino_param_set_algo(void* param, int algo) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  if (prm->plan.t2 == 0) return WINO_ERR_UNSUPPORTED;
+  return set_status(prm, replan(prm, algo, prm->plan.precision));
+}
+
+int wino_param_set_precision(void* param, int precision) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  if (prm->plan.t2 == 0) return WINO_ERR_UNSUPPORTED;
+  return set_status(prm, replan(prm, prm->plan.algo, precision));
+}
+
+int wino_param_set_stream(void* param, void* stream) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  prm->stream = (hipStream_t)stream;
+  return WINO_OK;
+}
+
+int wino_param_set_resident_filter(void* param, int on) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  prm->resident = on ? 1 : 0;
+  prm->filter_ready = 0;
+  return WINO_OK;
+}
+
+int wino_param_prepare_filter(void* param) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  if (prm->plan.t2 == 0) return WINO_ERR_UNSUPPORTED;
+  int st = stage_filter(prm);
+  if (!st) prm->filter_ready = 1;
+  return set_status(prm, st);
+}
+
+int wino_param_set_tensors(void* param, void* in, void* weight, void* out) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  if (weight != (void*)prm->weight) prm->filter_ready = 0;
+  prm->in = (_Float16*)in;
+  prm->weight = (_Float16*)weight;
+  prm->out = (_Float16*)out;
+  return WINO_OK;
+}
+
+int wino_param_destroy(void* param) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  if (prm->ws) (void)hipFree(prm->ws);
+  prm->ws = nullptr;
+  prm->magic = 0;
+  return WINO_OK;
+}
+
+int wino_param_info(const void* param, wino_info_t* info) {
+  Param* prm = as_param(param);
+  if (!prm || !info) return WINO_ERR_INVALID;
+  const Plan& pl = prm->plan;
+  memset(info, 0, sizeof(*info));
+  info->algo = pl.algo;
+  info->precision = pl.precision;
+  info->m = pl.m;
+  info->t2 = pl.t2;
+  info->oh = pl.oh;
+  info->ow = pl.ow;
+  info->tiles_h = pl.tiles_h;
+  info->tiles_w = pl.tiles_w;
+  info->tiles = pl.P;
+  info->kp = pl.Kp;
+  info->cp = pl.Cp;
+  info->pp = pl.Pp;
+  info->gemm_bk = pl.bk;
+  info->gemm_bp = pl.bp;
+  info->ws_bytes = pl.ws_bytes;
+  info->u_bytes = pl.u_bytes;
+  info->v_bytes = pl.v_bytes;
+  info->m_bytes = pl.m_bytes;
+  info->scale_bytes = pl.scale_bytes;
+  if (prm->ws) {
+    info->u = wsp(prm, pl.u_off);
+    if (pl.v_bytes) info->v = wsp(prm, pl.v_off);
+    if (pl.m_bytes) info->mbuf = wsp(prm, pl.m_off);
+    if (pl.precision != WINO_PREC_FP16) {
+      info->u_scale = wsp(prm, pl.us_off);
+      info->v_scale = wsp(prm, pl.vs_off);
+    }
+  }
+  return WINO_OK;
+}
+
+static int stage_guard(const void* param, Param** out) {
+  Param* prm = as_param(param);
+  if (!prm) return WINO_ERR_INVALID;
+  if (prm->plan.t2 == 0 || !prm->ws) return WINO_ERR_UNSUPPORTED;
+  *out = prm;
+  return WINO_OK;
+}
+
+int wino_stage_filter(const void* param) {
+  Param* prm;
+  int st = stage_guard(param, &prm);
+  return st ? st : set_status(prm, stage_filter(prm));
+}
+
+int wino_stage_input(const void* param) {
+  Param* prm;
+  int st = stage_guard(param, &prm);
+  if (st) return st;
+  if (prm->plan.algo == WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
+  return set_status(prm, stage_input(prm));
+}
+
+int wino_stage_gemm(const void* param) {
+  Param* prm;
+  int st = stage_guard(param, &prm);
+  if (st) return st;
+  if (prm->plan.algo == WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
+  return set_status(prm, stage_gemm(prm));
+}
+
+int wino_stage_output(const void* param) {
+  Param* prm;
+  int st = stage_guard(param, &prm);
+  if (st) return st;
+  if (prm->plan.algo == WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
+  return set_status(prm, stage_output(prm));
+}
+
+int wino_device_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return WINO_ERR_INVALID;
+  return hipMalloc(ptr, bytes) == hipSuccess ? WINO_OK : WINO_ERR_ALLOC;
+}
+int wino_device_free(void* ptr) { return hipFree(ptr) == hipSuccess ? WINO_OK : WINO_ERR_LAUNCH; }
+int wino_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+int wino_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+int wino_device_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
+
+}  // extern "C"

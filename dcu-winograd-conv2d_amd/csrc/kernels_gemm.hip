@@ -1,0 +1,320 @@
+// kernels_gemm.hip -- the T*T independent GEMMs of the non-fused pipeline
+// as one batched MFMA launch:  M[t] (K x P) = U[t] (K x C) . V[t] (C x P).
+//
+// Replaces the reference's hep_sgemm and its MMAC kernels
+// (include/hep_sgemm.h:495-612, K10-K13; call sites
+// src/winograd_2x3_non_fused.cpp:403-413 and src/winograd_4x3_non_fused.cpp:587-597):
+// same contraction and batch, but with this library's blocked operand layout
+// so every MFMA fragment is one 16- (fp16) or 32-byte (fp8) LDS read, LDS
+// double buffering fed by register prefetch, FP32 accumulation, and an
+// LDS-staged epilogue that writes M with full 16-byte stores.
+//
+// fp16 : v_mfma_f32_16x16x32_f16, reduction step 32 channels.
+// fp8  : v_mfma_scale_f32_16x16x128_f8f6f4 (e4m3 x e4m3), step 128 channels;
+//        the hi/lo split issues hi*hi + hi*lo + lo*hi (+ lo*lo for F(4x3),
+//        whose output transform amplifies the dropped term past 1e-2) with
+//        the lo operand's 2^-4 folded into the MFMA's E8M0 block scale
+//        (123 = 2^-4, 119 = 2^-8, 127 = 1).
+//        The epilogue multiplies by u_scale[t][k] * v_scale[t].
+#include "wino_internal.h"
+#include "wino_math.h"
+
+namespace wino {
+
+// Block = WK x WP waves; wave tile = (TK*16) x (TP*16); block tile BK x BP.
+template <int WK, int WP, int TK, int TP>
+__global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restrict__ U,
+                                                          const half8* __restrict__ V,
+                                                          half_t* __restrict__ M, uint32_t Kp,
+                                                          uint64_t Pp, uint32_t C8p) {
+  constexpr int NT = WK * WP * 64;
+  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+  constexpr int ALD = (4 * BK + NT - 1) / NT, BLD = (4 * BP + NT - 1) / NT;
+  constexpr int CS = BP + 8;  // epilogue row stride (halves)
+  constexpr int OPER = 2 * 4 * (BK + BP) * 16;
+  constexpr int EPI = BK * CS * 2;
+  __shared__ __attribute__((aligned(16))) char smem[OPER > EPI ? OPER : EPI];
+  half8* As = (half8*)smem;                 // [2][4][BK]
+  half8* Bs = (half8*)smem + 2 * 4 * BK;    // [2][4][BP]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wk = wave % WK, wp = wave / WK;
+  const uint32_t t = blockIdx.z;
+  const uint32_t k0 = blockIdx.y * BK;
+  const uint64_t p0 = (uint64_t)blockIdx.x * BP;
+  const half8* Ut = U + (uint64_t)t * C8p * Kp + k0;
+  const half8* Vt = V + (uint64_t)t * C8p * Pp + p0;
+  const int nsteps = (int)(C8p / 4);
+
+  half8 ra[ALD], rb[BLD];
+  auto gload = [&](int s) {
+#pragma unroll
+    for (int i = 0; i < ALD; ++i) {
+      const int q = tid + i * NT;
+      if (q < 4 * BK) ra[i] = Ut[(uint64_t)(4 * s + q / BK) * Kp + (q % BK)];
+    }
+#pragma unroll
+    for (int i = 0; i < BLD; ++i) {
+      const int q = tid + i * NT;
+      if (q < 4 * BP) rb[i] = Vt[(uint64_t)(4 * s + q / BP) * Pp + (q % BP)];
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < ALD; ++i) {
+      const int q = tid + i * NT;
+      if (q < 4 * BK) As[buf * 4 * BK + q] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < BLD; ++i) {
+      const int q = tid + i * NT;
+      if (q < 4 * BP) Bs[buf * 4 * BP + q] = rb[i];
+    }
+  };
+
+  floatx4 acc[TK][TP];
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const int g = lane >> 4, r16 = lane & 15;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nsteps) gload(s + 1);
+    half8 a[TK], b[TP];
+#pragma unroll
+    for (int i = 0; i < TK; ++i) a[i] = As[(cur * 4 + g) * BK + wk * TK * 16 + i * 16 + r16];
+#pragma unroll
+    for (int j = 0; j < TP; ++j) b[j] = Bs[(cur * 4 + g) * BP + wp * TP * 16 + j * 16 + r16];
+#pragma unroll
+    for (int i = 0; i < TK; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (s + 1 < nsteps) sstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue: acc -> fp16 tile in LDS -> 16-byte row stores of M
+  half_t* Cs = (half_t*)smem;
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int j = 0; j < TP; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = wk * TK * 16 + i * 16 + 4 * g + e;
+        const int col = wp * TP * 16 + j * 16 + r16;
+        Cs[row * CS + col] = (half_t)acc[i][j][e];
+      }
+  __syncthreads();
+  half_t* Mt = M + ((uint64_t)t * Kp + k0) * Pp + p0;
+  constexpr int CHUNKS = BK * BP / 8;
+  for (int q = tid; q < CHUNKS; q += NT) {
+    const int row = q / (BP / 8), c8 = q % (BP / 8);
+    *(half8*)(Mt + (uint64_t)row * Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
+  }
+}
+
+// fp8 operands: 32-byte groups [t][c32][rows][32]; PLANES 1 (e4m3) or 2
+// (hi plane followed by lo plane, each T2*C32p*rows*32 bytes).
+template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
+__global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
+    const intx8* __restrict__ U, const intx8* __restrict__ V, half_t* __restrict__ M,
+    const float* __restrict__ us, const float* __restrict__ vs, uint32_t Kp, uint64_t Pp,
+    uint32_t C32p, uint32_t T2) {
+  constexpr int NT = WK * WP * 64;
+  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+  constexpr int ALD = (PLANES * 4 * BK + NT - 1) / NT, BLD = (PLANES * 4 * BP + NT - 1) / NT;
+  constexpr int CS = BP + 8;
+  constexpr int EPI = BK * CS * 2;
+  extern __shared__ __attribute__((aligned(16))) char dsmem[];
+  (void)EPI;
+  intx8* As = (intx8*)dsmem;                          // [2][PLANES][4][BK]
+  intx8* Bs = (intx8*)dsmem + 2 * PLANES * 4 * BK;    // [2][PLANES][4][BP]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wk = wave % WK, wp = wave / WK;
+  const uint32_t t = blockIdx.z;
+  const uint32_t k0 = blockIdx.y * BK;
+  const uint64_t p0 = (uint64_t)blockIdx.x * BP;
+  const uint64_t uplane = (uint64_t)T2 * C32p * Kp;  // intx8 entries per plane
+  const uint64_t vplane = (uint64_t)T2 * C32p * Pp;
+  const intx8* Ut = U + (uint64_t)t * C32p * Kp + k0;
+  const intx8* Vt = V + (uint64_t)t * C32p * Pp + p0;
+  const int nsteps = (int)(C32p / 4);
+
+  intx8 ra[ALD], rb[BLD];
+  auto gload = [&](int s) {
+#pragma unroll
+    for (int i = 0; i < ALD; ++i) {
+      const int q = tid + i * NT;
+      if (q < PLANES * 4 * BK) {
+        const int pl = q / (4 * BK), qq = q % (4 * BK);
+        ra[i] = Ut[pl * uplane + (uint64_t)(4 * s + qq / BK) * Kp + (qq % BK)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < BLD; ++i) {
+      const int q = tid + i * NT;
+      if (q < PLANES * 4 * BP) {
+        const int pl = q / (4 * BP), qq = q % (4 * BP);
+        rb[i] = Vt[pl * vplane + (uint64_t)(4 * s + qq / BP) * Pp + (qq % BP)];
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < ALD; ++i) {
+      const int q = tid + i * NT;
+      if (q < PLANES * 4 * BK) As[buf * PLANES * 4 * BK + q] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < BLD; ++i) {
+      const int q = tid + i * NT;
+      if (q < PLANES * 4 * BP) Bs[buf * PLANES * 4 * BP + q] = rb[i];
+    }
+  };
+
+  floatx4 acc[TK][TP];
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const int g = lane >> 4, r16 = lane & 15;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nsteps) gload(s + 1);
+    intx8 a[PLANES][TK], b[PLANES][TP];
+#pragma unroll
+    for (int pl = 0; pl < PLANES; ++pl) {
+#pragma unroll
+      for (int i = 0; i < TK; ++i)
+        a[pl][i] = As[((cur * PLANES + pl) * 4 + g) * BK + wk * TK * 16 + i * 16 + r16];
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+        b[pl][j] = Bs[((cur * PLANES + pl) * 4 + g) * BP + wp * TP * 16 + j * 16 + r16];
+    }
+#pragma unroll
+    for (int i = 0; i < TK; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[0][i], b[0][j], acc[i][j],
+                                                                     0, 0, 0, 127, 0, 127);
+        if (PLANES == 2) {
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+              a[0][i], b[PLANES - 1][j], acc[i][j], 0, 0, 0, 127, 0, 123);
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+              a[PLANES - 1][i], b[0][j], acc[i][j], 0, 0, 0, 123, 0, 127);
+          if (LOLO)
+            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                a[PLANES - 1][i], b[PLANES - 1][j], acc[i][j], 0, 0, 0, 123, 0, 123);
+        }
+      }
+    if (s + 1 < nsteps) sstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  half_t* Cs = (half_t*)dsmem;
+  const float sv = vs[t];
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = wk * TK * 16 + i * 16 + 4 * g + e;
+      const float sc = us[(uint64_t)t * Kp + k0 + row] * sv;
+#pragma unroll
+      for (int j = 0; j < TP; ++j) {
+        const int col = wp * TP * 16 + j * 16 + r16;
+        Cs[row * CS + col] = (half_t)(acc[i][j][e] * sc);
+      }
+    }
+  __syncthreads();
+  half_t* Mt = M + ((uint64_t)t * Kp + k0) * Pp + p0;
+  constexpr int CHUNKS = BK * BP / 8;
+  for (int q = tid; q < CHUNKS; q += NT) {
+    const int row = q / (BP / 8), c8 = q % (BP / 8);
+    *(half8*)(Mt + (uint64_t)row * Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
+  }
+}
+
+// ------------------------------------------------------------ tile table
+// Index -> (WK, WP, TK, TP).  The planner picks an entry per problem
+// (planner.cpp); every entry is instantiated below for both precisions.
+static const int kCfg[][4] = {
+    {2, 2, 4, 4},  // 0: 128 x 128, 256 threads
+    {2, 2, 2, 4},  // 1:  64 x 128
+    {2, 2, 2, 2},  // 2:  64 x  64
+    {1, 4, 2, 2},  // 3:  32 x 128
+    {1, 2, 1, 2},  // 4:  16 x  64, 128 threads
+};
+int gemm_cfg_count() { return (int)(sizeof(kCfg) / sizeof(kCfg[0])); }
+GemmCfg gemm_cfg(int idx) {
+  GemmCfg c;
+  c.bk = kCfg[idx][0] * kCfg[idx][2] * 16;
+  c.bp = kCfg[idx][1] * kCfg[idx][3] * 16;
+  c.threads = kCfg[idx][0] * kCfg[idx][1] * 64;
+  return c;
+}
+
+template <int WK, int WP, int TK, int TP>
+static void run_f16(const Plan& pl, const void* U, const void* V, half_t* M, hipStream_t st) {
+  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+  dim3 grid((uint32_t)(pl.Pp / BP), pl.Kp / BK, pl.t2);
+  hipLaunchKernelGGL((k_gemm_f16<WK, WP, TK, TP>), grid, dim3(WK * WP * 64), 0, st,
+                     (const half8*)U, (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8);
+}
+
+template <int WK, int WP, int TK, int TP>
+static void run_f8(const Plan& pl, const void* U, const void* V, half_t* M, const float* us,
+                   const float* vs, hipStream_t st) {
+  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+  dim3 grid((uint32_t)(pl.Pp / BP), pl.Kp / BK, pl.t2);
+  const size_t oper1 = 2 * 1 * 4 * (BK + BP) * 32, oper2 = 2 * 2 * 4 * (BK + BP) * 32;
+  const size_t epi = (size_t)BK * (BP + 8) * 2;
+  if (pl.planes == 2) {
+    auto kern = pl.m == 4 ? k_gemm_f8<WK, WP, TK, TP, 2, true> : k_gemm_f8<WK, WP, TK, TP, 2, false>;
+    const size_t lds = oper2 > epi ? oper2 : epi;
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, grid, dim3(WK * WP * 64), lds, st, (const intx8*)U, (const intx8*)V,
+                       M, us, vs, pl.Kp, pl.Pp, pl.Cp / 32, (uint32_t)pl.t2);
+  } else {
+    auto kern = k_gemm_f8<WK, WP, TK, TP, 1, false>;
+    const size_t lds = oper1 > epi ? oper1 : epi;
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, grid, dim3(WK * WP * 64), lds, st, (const intx8*)U, (const intx8*)V,
+                       M, us, vs, pl.Kp, pl.Pp, pl.Cp / 32, (uint32_t)pl.t2);
+  }
+}
+
+hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M,
+                       const float* u_scale, const float* v_scale, hipStream_t st) {
+  if (pl.precision == WINO_PREC_FP16) {
+    switch (pl.gemm_cfg) {
+      case 0: run_f16<2, 2, 4, 4>(pl, U, V, M, st); break;
+      case 1: run_f16<2, 2, 2, 4>(pl, U, V, M, st); break;
+      case 2: run_f16<2, 2, 2, 2>(pl, U, V, M, st); break;
+      case 3: run_f16<1, 4, 2, 2>(pl, U, V, M, st); break;
+      default: run_f16<1, 2, 1, 2>(pl, U, V, M, st); break;
+    }
+  } else {
+    switch (pl.gemm_cfg) {
+      case 0: run_f8<2, 2, 4, 4>(pl, U, V, M, u_scale, v_scale, st); break;
+      case 1: run_f8<2, 2, 2, 4>(pl, U, V, M, u_scale, v_scale, st); break;
+      case 2: run_f8<2, 2, 2, 2>(pl, U, V, M, u_scale, v_scale, st); break;
+      case 3: run_f8<1, 4, 2, 2>(pl, U, V, M, u_scale, v_scale, st); break;
+      default: run_f8<1, 2, 1, 2>(pl, U, V, M, u_scale, v_scale, st); break;
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace wino

@@ -1,0 +1,83 @@
+// wino_internal.h -- shared host/device definitions of the MI355X Winograd
+// conv2d operator: the parameter blob, the execution plan, and the launchers
+// each kernel file provides.
+//
+// Data layout in HBM (all written by this library; pads are zero):
+//   U  fp16  [t2][cp/8][kp][8]    transformed filter, 8-channel groups so a
+//                                 16x16x32 MFMA operand is one 16-byte load
+//   V  fp16  [t2][cp/8][pp][8]    transformed input, same grouping
+//   M  fp16  [t2][kp][pp]         GEMM result, tile index p contiguous
+//   U8 e4m3  [planes][t2][cp/32][kp][32]  fp8 precisions (planes = 1 or 2)
+//   V8 e4m3  [planes][t2][cp/32][pp][32]
+//   u_scale  f32 [t2][kp], v_scale f32 [t2], amax f32 [1]
+// Tile p enumerates (n, th, tw) row-major as the reference does
+// (include/common.h:113-120).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/winoconv.h"
+
+namespace wino {
+
+constexpr uint64_t kMagic = 0x57494e4f43563031ull;  // "WINOCV01"
+
+struct Plan {
+  int algo;        // WINO_ALGO_*
+  int precision;   // WINO_PREC_*
+  int m, T, t2;    // output tile edge, input tile edge, T*T
+  uint32_t oh, ow;
+  uint32_t tiles_h, tiles_w;
+  uint64_t P;      // n * tiles_h * tiles_w
+  uint32_t Kp, Cp;
+  uint64_t Pp;
+  int gemm_cfg;    // index into the GEMM tile table
+  uint32_t bk, bp; // GEMM tile rows / columns
+  int planes;      // 1, or 2 for the fp8 hi/lo split
+  uint64_t u_off, v_off, m_off, us_off, vs_off, amax_off;
+  uint64_t u_bytes, v_bytes, m_bytes, scale_bytes;
+  uint64_t ws_bytes;
+};
+
+// The opaque blob handed out by getParamsize/getkernelInfo.  Plain data:
+// callers malloc/free it (reference src/main.cpp:80,116).
+struct Param {
+  uint64_t magic;
+  _Float16* in;
+  _Float16* weight;
+  _Float16* out;
+  uint32_t n, c, h, w, k, r, s, u, v, p, q;
+  int32_t resident;      // reuse U across calls
+  int32_t filter_ready;  // U holds the transform of `weight`
+  int32_t status;        // last WINO_* status
+  int32_t reserved;
+  hipStream_t stream;
+  void* ws;              // device workspace (owned)
+  Plan plan;
+};
+
+// ------------------------------------------------------------ GEMM table
+struct GemmCfg {
+  int bk, bp;      // block tile
+  int threads;
+};
+int gemm_cfg_count();
+GemmCfg gemm_cfg(int idx);
+
+// ------------------------------------------------------------ launchers
+// All return hipSuccess or the launch error; they only enqueue.
+hipError_t launch_filter_transform(const Plan& pl, const _Float16* w, uint32_t K, uint32_t C,
+                                   void* U, float* u_scale, hipStream_t st);
+hipError_t launch_input_amax(const _Float16* x, uint64_t count, float* amax, hipStream_t st);
+hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N, uint32_t C,
+                                  uint32_t H, uint32_t W, void* V, const float* amax,
+                                  float* v_scale, hipStream_t st);
+hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M,
+                       const float* u_scale, const float* v_scale, hipStream_t st);
+hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N, uint32_t K,
+                                   _Float16* y, hipStream_t st);
+hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, uint32_t N,
+                            uint32_t C, uint32_t H, uint32_t W, uint32_t K, _Float16* y,
+                            hipStream_t st);
+
+}  // namespace wino

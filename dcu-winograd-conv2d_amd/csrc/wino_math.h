@@ -1,0 +1,191 @@
+// wino_math.h -- Winograd coefficient application (FP32) and the e4m3
+// quantizer, shared by the transform, GEMM and fused kernels.
+//
+// Matrices (the standard F(2,3) / F(4,3) ones of Lavin & Gray 2016, as used
+// by the reference):
+//   F(2x3) G, B^T, A^T: src/winograd_2x3_non_fused.cpp:120-160, 324-377, 184-227
+//   F(4x3) G, B^T, A^T: src/winograd_4x3_non_fused.cpp:342-404, 41-146, 427-525
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wino {
+
+typedef _Float16 half_t;
+typedef half_t half8 __attribute__((ext_vector_type(8)));
+typedef half_t half4 __attribute__((ext_vector_type(4)));
+typedef half_t half2_t __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
+typedef int intx4 __attribute__((ext_vector_type(4)));
+
+template <int m>
+struct Tf;
+
+template <>
+struct Tf<2> {
+  static constexpr int T = 4;
+  // z = G g (one 3-vector -> 4)
+  __device__ static inline void G(const float* g, float* z) {
+    z[0] = g[0];
+    z[1] = 0.5f * (g[0] + g[1] + g[2]);
+    z[2] = 0.5f * (g[0] - g[1] + g[2]);
+    z[3] = g[2];
+  }
+  // z = B^T d (4 -> 4)
+  __device__ static inline void BT(const float* d, float* z) {
+    z[0] = d[0] - d[2];
+    z[1] = d[1] + d[2];
+    z[2] = d[2] - d[1];
+    z[3] = d[1] - d[3];
+  }
+  // y = A^T x (4 -> 2)
+  __device__ static inline void AT(const float* x, float* y) {
+    y[0] = x[0] + x[1] + x[2];
+    y[1] = x[1] - x[2] - x[3];
+  }
+  // sum_j |B^T[i][j]|: worst-case gain of one 1-D input transform row
+  __host__ __device__ static constexpr float gain(int) { return 2.0f; }
+};
+
+template <>
+struct Tf<4> {
+  static constexpr int T = 6;
+  __device__ static inline void G(const float* g, float* z) {
+    const float s = g[0] + g[2];
+    z[0] = 0.25f * g[0];
+    z[1] = -(s + g[1]) * (1.0f / 6.0f);
+    z[2] = -(s - g[1]) * (1.0f / 6.0f);
+    const float a = g[0] * (1.0f / 24.0f) + g[2] * (1.0f / 6.0f);
+    z[3] = a + g[1] * (1.0f / 12.0f);
+    z[4] = a - g[1] * (1.0f / 12.0f);
+    z[5] = g[2];
+  }
+  __device__ static inline void BT(const float* d, float* z) {
+    z[0] = 4.0f * d[0] - 5.0f * d[2] + d[4];
+    z[1] = -4.0f * (d[1] + d[2]) + d[3] + d[4];
+    z[2] = 4.0f * (d[1] - d[2]) - d[3] + d[4];
+    z[3] = -2.0f * (d[1] - d[3]) - d[2] + d[4];
+    z[4] = 2.0f * (d[1] - d[3]) - d[2] + d[4];
+    z[5] = 4.0f * d[1] - 5.0f * d[3] + d[5];
+  }
+  __device__ static inline void AT(const float* x, float* y) {
+    const float a = x[1] + x[2], b = x[1] - x[2], c = x[3] + x[4], d = x[3] - x[4];
+    y[0] = x[0] + a + c;
+    y[1] = b + 2.0f * d;
+    y[2] = a + 4.0f * c;
+    y[3] = b + 8.0f * d + x[5];
+  }
+  __host__ __device__ static constexpr float gain(int i) {
+    return (i == 3 || i == 4) ? 6.0f : 10.0f;
+  }
+};
+
+// u[a*T+b] = (G g G^T)[a][b], g row-major 3x3.
+template <int m>
+__device__ inline void filter2d(const float* g, float* u) {
+  constexpr int T = m + 2;
+  float tmp[T][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    float col[3] = {g[j], g[3 + j], g[6 + j]}, z[T];
+    Tf<m>::G(col, z);
+#pragma unroll
+    for (int a = 0; a < T; ++a) tmp[a][j] = z[a];
+  }
+#pragma unroll
+  for (int a = 0; a < T; ++a) {
+    float z[T];
+    Tf<m>::G(tmp[a], z);
+#pragma unroll
+    for (int b = 0; b < T; ++b) u[a * T + b] = z[b];
+  }
+}
+
+// v[a*T+b] = (B^T d B)[a][b], d row-major T x T.
+template <int m>
+__device__ inline void input2d(const float* d, float* v) {
+  constexpr int T = m + 2;
+  float tmp[T][T];
+#pragma unroll
+  for (int j = 0; j < T; ++j) {
+    float col[T], z[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) col[i] = d[i * T + j];
+    Tf<m>::BT(col, z);
+#pragma unroll
+    for (int a = 0; a < T; ++a) tmp[a][j] = z[a];
+  }
+#pragma unroll
+  for (int a = 0; a < T; ++a) {
+    float z[T];
+    Tf<m>::BT(tmp[a], z);
+#pragma unroll
+    for (int b = 0; b < T; ++b) v[a * T + b] = z[b];
+  }
+}
+
+// y[i*m+j] = (A^T M A)[i][j], x row-major T x T.
+template <int m>
+__device__ inline void output2d(const float* x, float* y) {
+  constexpr int T = m + 2;
+  float tmp[m][T];
+#pragma unroll
+  for (int b = 0; b < T; ++b) {
+    float col[T], z[m];
+#pragma unroll
+    for (int a = 0; a < T; ++a) col[a] = x[a * T + b];
+    Tf<m>::AT(col, z);
+#pragma unroll
+    for (int i = 0; i < m; ++i) tmp[i][b] = z[i];
+  }
+#pragma unroll
+  for (int i = 0; i < m; ++i) {
+    float z[m];
+    Tf<m>::AT(tmp[i], z);
+#pragma unroll
+    for (int j = 0; j < m; ++j) y[i * m + j] = z[j];
+  }
+}
+
+// OCP FP8 E4M3 (bias 7, max 448, no infinities): round to nearest even,
+// saturating to +-448.  Written out in integer arithmetic so that the host
+// oracle's quantizer and this one agree bit for bit.
+__device__ inline uint32_t f32_to_e4m3(float x) {
+  const uint32_t bits = __float_as_uint(x);
+  const uint32_t sign = (bits >> 24) & 0x80u;
+  const float a = fabsf(x);
+  if (!(a < 448.0f)) return sign | 0x7eu;
+  if (a < 0.015625f) {  // below 2^-6: subnormal, quantum 2^-9
+    return sign | (uint32_t)rintf(a * 512.0f);
+  }
+  const uint32_t ab = __float_as_uint(a);
+  int e = (int)(ab >> 23) - 127;
+  uint32_t keep = (ab >> 20) & 7u;
+  const uint32_t rest = ab & 0xfffffu;
+  if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) ++keep;
+  if (keep == 8u) {
+    keep = 0;
+    ++e;
+  }
+  return sign | ((uint32_t)(e + 7) << 3) | keep;
+}
+
+__device__ inline float e4m3_to_f32(uint32_t b) {
+  const int e = (b >> 3) & 15, f = b & 7;
+  const float v = e ? ldexpf((float)(8 + f), e - 10) : ldexpf((float)f, -9);
+  return (b & 0x80u) ? -v : v;
+}
+
+// Smallest power of two s with amax / s <= 448 (1 for amax == 0).
+__host__ __device__ inline float pow2_scale(float amax) {
+  if (!(amax > 0.0f)) return 1.0f;
+  int e;
+  frexpf(amax / 448.0f, &e);  // amax/448 = f * 2^e, f in [0.5, 1)
+  float s = ldexpf(1.0f, e);
+  if (amax / s > 448.0f) s *= 2.0f;
+  if (amax / (s * 0.5f) <= 448.0f) s *= 0.5f;
+  return s;
+}
+
+}  // namespace wino

@@ -1,0 +1,208 @@
+"""ctypes binding of libwinoconv.so, the MI355X Winograd conv2d operator.
+
+Mirrors the reference operator interface (include/conv2d.h:8-42 and the
+harness flow of src/main.cpp:78-105): fill a ``problem_t``, size a parameter
+blob with ``getParamsize``, set it up with ``getkernelInfo``, call
+``conv2d_fp16``.  The extended controls of include/winoconv.h (algorithm
+override, precision, stream, resident filters, stage entry points, grouped
+launches, serving) are bound here too.
+
+Device memory is passed as integer device addresses (for example
+``torch.Tensor.data_ptr()``); this module never allocates device memory
+except through the explicit ``device_*`` helpers.  Loading fails loudly when
+the library has not been built: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwinoconv.so")
+
+# status codes (include/winoconv.h)
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_ALLOC, ERR_LAUNCH, ERR_CAPACITY, ERR_BUSY, ERR_NOT_FOUND = range(8)
+# algorithms; 0..2 are the reference's winograd_select values (include/common.h:140-144)
+ALGO_AUTO, ALGO_F43, ALGO_F23, ALGO_F23_FUSED, ALGO_F43_FUSED = -1, 0, 1, 2, 3
+ALGO_NAMES = {ALGO_F43: "f43", ALGO_F23: "f23", ALGO_F23_FUSED: "f23_fused", ALGO_F43_FUSED: "f43_fused"}
+# GEMM operand precisions
+PREC_FP16, PREC_FP8, PREC_FP8X2 = 0, 1, 2
+PREC_NAMES = {PREC_FP16: "fp16", PREC_FP8: "fp8", PREC_FP8X2: "fp8x2"}
+
+
+class Problem(C.Structure):
+    """problem_t, include/conv2d.h:8-24."""
+    _fields_ = [("in_", C.c_void_p), ("weight", C.c_void_p), ("out", C.c_void_p)] + [
+        (f, C.c_uint) for f in ("n", "c", "h", "w", "k", "r", "s", "u", "v", "p", "q")]
+
+
+class KernelInfo(C.Structure):
+    """kernelInfo_t, include/conv2d.h:26-36."""
+    _fields_ = [(f, C.c_uint) for f in ("blockx", "blocky", "blockz", "threadx", "thready",
+                                          "threadz", "dynmicLdsSize")] + [("kernelPtr", C.c_void_p)]
+
+
+class Info(C.Structure):
+    """wino_info_t, include/winoconv.h."""
+    _fields_ = [("algo", C.c_int), ("precision", C.c_int), ("m", C.c_int), ("t2", C.c_int),
+                ("oh", C.c_uint32), ("ow", C.c_uint32), ("tiles_h", C.c_uint32),
+                ("tiles_w", C.c_uint32), ("tiles", C.c_uint64), ("kp", C.c_uint32),
+                ("cp", C.c_uint32), ("pp", C.c_uint64), ("gemm_bk", C.c_uint32),
+                ("gemm_bp", C.c_uint32), ("ws_bytes", C.c_uint64), ("u_bytes", C.c_uint64),
+                ("v_bytes", C.c_uint64), ("m_bytes", C.c_uint64), ("scale_bytes", C.c_uint64),
+                ("u", C.c_void_p), ("v", C.c_void_p), ("mbuf", C.c_void_p),
+                ("u_scale", C.c_void_p), ("v_scale", C.c_void_p)]
+
+
+# name -> (restype, argtypes); every symbol include/*.h declares
+_P = C.c_void_p
+_SIGS = {
+    "getParamsize": (C.c_int, [C.POINTER(Problem), C.POINTER(C.c_int)]),
+    "getkernelInfo": (C.c_int, [C.POINTER(Problem), C.POINTER(KernelInfo), _P]),
+    "conv2d_fp16": (None, [_P]),
+    "wino_status_string": (C.c_char_p, [C.c_int]),
+    "wino_select_algo": (C.c_int, [C.POINTER(Problem)]),
+    "wino_validate": (C.c_int, [C.POINTER(Problem)]),
+    "wino_param_set_algo": (C.c_int, [_P, C.c_int]),
+    "wino_param_set_precision": (C.c_int, [_P, C.c_int]),
+    "wino_param_set_stream": (C.c_int, [_P, _P]),
+    "wino_param_set_resident_filter": (C.c_int, [_P, C.c_int]),
+    "wino_param_prepare_filter": (C.c_int, [_P]),
+    "wino_param_set_tensors": (C.c_int, [_P, _P, _P, _P]),
+    "wino_run": (C.c_int, [_P]),
+    "wino_param_status": (C.c_int, [_P]),
+    "wino_param_destroy": (C.c_int, [_P]),
+    "wino_param_info": (C.c_int, [_P, C.POINTER(Info)]),
+    "wino_stage_filter": (C.c_int, [_P]),
+    "wino_stage_input": (C.c_int, [_P]),
+    "wino_stage_gemm": (C.c_int, [_P]),
+    "wino_stage_output": (C.c_int, [_P]),
+    "wino_device_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "wino_device_free": (C.c_int, [_P]),
+    "wino_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
+    "wino_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+    "wino_device_synchronize": (C.c_int, []),
+}
+
+_lib = None
+
+
+class WinoError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})" if what else status_string(status))
+
+
+def build() -> None:
+    """Compile libwinoconv.so in-tree (hipcc, gfx950)."""
+    jobs = str(min(8, os.cpu_count() or 1))
+    subprocess.run(["make", "-s", "-j" + jobs, "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is not built; run `make -C {HERE}` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(st: int) -> str:
+    return lib().wino_status_string(st).decode()
+
+
+def check(st: int, what: str = "") -> None:
+    if st != OK:
+        raise WinoError(st, what)
+
+
+def make_problem(x, w, y, n, c, h, wd, k, r=3, s=3, u=1, v=1, p=1, q=1) -> Problem:
+    return Problem(x or None, w or None, y or None, n, c, h, wd, k, r, s, u, v, p, q)
+
+
+def select_algo(n, c, h, w, k) -> int:
+    """The reference selection rule (src/conv2d.cpp:83-114) for a 3x3 s1 p1 conv."""
+    pr = make_problem(0, 0, 0, n, c, h, w, k)
+    return lib().wino_select_algo(C.byref(pr))
+
+
+class Conv2d:
+    """One configured convolution: the reference's param blob plus controls.
+
+    x, w, y are device addresses of NCHW input, KCRS filter and NKHW output.
+    """
+
+    def __init__(self, x: int, w: int, y: int, n: int, c: int, h: int, wd: int, k: int,
+                 algo: int = ALGO_AUTO, precision: int = PREC_FP16, resident: bool = False,
+                 stream: int | None = None):
+        L = lib()
+        self.problem = make_problem(x, w, y, n, c, h, wd, k)
+        size = C.c_int(0)
+        check(L.getParamsize(C.byref(self.problem), C.byref(size)), "getParamsize")
+        self._blob = C.create_string_buffer(size.value)
+        self.kinfo = KernelInfo()
+        check(L.getkernelInfo(C.byref(self.problem), C.byref(self.kinfo), self._blob), "getkernelInfo")
+        self._live = True
+        try:
+            if algo != ALGO_AUTO and algo != self.info().algo:
+                check(L.wino_param_set_algo(self._blob, algo), "set_algo")
+            if precision != PREC_FP16:
+                check(L.wino_param_set_precision(self._blob, precision), "set_precision")
+            if resident:
+                check(L.wino_param_set_resident_filter(self._blob, 1), "set_resident")
+            if stream:
+                check(L.wino_param_set_stream(self._blob, stream), "set_stream")
+        except Exception:
+            self.destroy()
+            raise
+
+    @property
+    def param(self):
+        return self._blob
+
+    def info(self) -> Info:
+        inf = Info()
+        check(lib().wino_param_info(self._blob, C.byref(inf)), "info")
+        return inf
+
+    def __call__(self) -> None:
+        """conv2d_fp16 through the status-returning entry point."""
+        check(lib().wino_run(self._blob), "wino_run")
+
+    def run_reference_entry(self) -> None:
+        """The reference entry point itself (void conv2d_fp16)."""
+        lib().conv2d_fp16(self._blob)
+        check(lib().wino_param_status(self._blob), "conv2d_fp16")
+
+    def set_tensors(self, x: int, w: int, y: int) -> None:
+        check(lib().wino_param_set_tensors(self._blob, x, w, y), "set_tensors")
+
+    def set_stream(self, stream: int | None) -> None:
+        check(lib().wino_param_set_stream(self._blob, stream or None), "set_stream")
+
+    def prepare_filter(self) -> None:
+        check(lib().wino_param_prepare_filter(self._blob), "prepare_filter")
+
+    def stage(self, name: str) -> None:
+        fn = {"filter": lib().wino_stage_filter, "input": lib().wino_stage_input,
+              "gemm": lib().wino_stage_gemm, "output": lib().wino_stage_output}[name]
+        check(fn(self._blob), "stage " + name)
+
+    def destroy(self) -> None:
+        if getattr(self, "_live", False):
+            lib().wino_param_destroy(self._blob)
+            self._live = False
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

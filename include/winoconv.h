@@ -1,0 +1,130 @@
+/*
+ * winoconv.h -- extended C ABI of the MI355X Winograd conv2d operator.
+ *
+ * These entry points have no counterpart in the reference API
+ * (include/conv2d.h:39-42); each comment names the reference behaviour it
+ * makes explicit or replaces.  Plain pointers and sizes only: device
+ * pointers are `void*`, streams are `void*` (a hipStream_t).
+ */
+#ifndef WINO_WINOCONV_H
+#define WINO_WINOCONV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "conv2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  WINO_OK = 0,
+  WINO_ERR_INVALID = 1,     /* null pointer, uninitialised blob, bad enum     */
+  WINO_ERR_UNSUPPORTED = 2, /* not 3x3 / stride 1 / pad 1, or zero extent     */
+  WINO_ERR_ALLOC = 3,       /* device allocation failed                       */
+  WINO_ERR_LAUNCH = 4,      /* kernel launch or HIP runtime error             */
+  WINO_ERR_CAPACITY = 5,    /* request exceeds a configured capacity          */
+  WINO_ERR_BUSY = 6,        /* result not ready yet (poll again)              */
+  WINO_ERR_NOT_FOUND = 7    /* unknown id / handle                            */
+};
+const char* wino_status_string(int status);
+
+/* ------------------------------------------------------------ algorithms */
+/* Values 0..2 equal the reference's winograd_select enum
+ * (include/common.h:140-144). */
+enum {
+  WINO_ALGO_AUTO = -1,     /* the reference selector, src/conv2d.cpp:83-114 */
+  WINO_ALGO_F43 = 0,       /* F(4x3) non-fused: 4 kernels                    */
+  WINO_ALGO_F23 = 1,       /* F(2x3) non-fused: 4 kernels                    */
+  WINO_ALGO_F23_FUSED = 2, /* F(2x3) fused: transform+GEMM+inverse in one    */
+  WINO_ALGO_F43_FUSED = 3  /* F(4x3) fused (no reference counterpart)        */
+};
+
+/* GEMM-stage operand precision.  Transforms are FP32 and the GEMM always
+ * accumulates in FP32. */
+enum {
+  WINO_PREC_FP16 = 0,  /* fp16 U and V, fp16 MFMA                              */
+  WINO_PREC_FP8 = 1,   /* e4m3 U (per-(t,k) scale) and V (per-t scale)         */
+  WINO_PREC_FP8X2 = 2  /* e4m3 hi+lo split of U and V, 3 MFMA products        */
+};
+
+/* Pure selector: the reference rule (src/conv2d.cpp:83-114) evaluated
+ * without the reference's process-global tile state.  Returns the algo. */
+int wino_select_algo(const problem_t* problem);
+
+/* Checks that the problem is one this operator computes (3x3, stride 1,
+ * pad 1, nonzero extents, sizes within 32-bit element counts).  The
+ * reference accepted anything and computed garbage (src/conv2d.cpp:24-114). */
+int wino_validate(const problem_t* problem);
+
+/* -------------------------------------------------- parameter-blob control */
+/* Re-plans an initialised blob for another algorithm / precision; the
+ * workspace is re-allocated when its size changes. */
+int wino_param_set_algo(void* param, int algo);
+int wino_param_set_precision(void* param, int precision);
+/* Stream for subsequent conv2d_fp16 / wino_run calls (NULL = null stream). */
+int wino_param_set_stream(void* param, void* stream);
+/* Resident filters: when on, the filter transform runs once (at the next
+ * call or wino_param_prepare_filter) and later calls reuse U.  Off (the
+ * default) re-transforms the filter on every call, as the reference does
+ * (src/winograd_4x3_non_fused.cpp:584). */
+int wino_param_set_resident_filter(void* param, int on);
+int wino_param_prepare_filter(void* param);
+/* Replaces the device pointers of the problem (same shape). */
+int wino_param_set_tensors(void* param, void* in, void* weight, void* out);
+/* Status-returning form of conv2d_fp16. */
+int wino_run(const void* param);
+/* Last status recorded in the blob. */
+int wino_param_status(const void* param);
+/* Frees the device workspace (the reference never freed it, D2). */
+int wino_param_destroy(void* param);
+
+typedef struct {
+  int algo;          /* WINO_ALGO_*                                   */
+  int precision;     /* WINO_PREC_*                                   */
+  int m;             /* output tile edge (2 or 4)                     */
+  int t2;            /* tile positions (16 or 36)                     */
+  uint32_t oh, ow;   /* output size                                   */
+  uint32_t tiles_h;  /* ceil(oh / m)                                  */
+  uint32_t tiles_w;  /* ceil(ow / m)                                  */
+  uint64_t tiles;    /* P = n * tiles_h * tiles_w                     */
+  uint32_t kp;       /* K padded to the GEMM row tile                 */
+  uint32_t cp;       /* C padded to the reduction step                */
+  uint64_t pp;       /* P padded to the GEMM column tile              */
+  uint32_t gemm_bk;  /* GEMM tile rows (k)                            */
+  uint32_t gemm_bp;  /* GEMM tile columns (p)                         */
+  uint64_t ws_bytes; /* device workspace bytes                        */
+  uint64_t u_bytes, v_bytes, m_bytes, scale_bytes;
+  void* u;           /* device pointers into the workspace            */
+  void* v;
+  void* mbuf;
+  void* u_scale;     /* float [t2][kp] (fp8 precisions)               */
+  void* v_scale;     /* float [t2]     (fp8 precisions)               */
+} wino_info_t;
+int wino_param_info(const void* param, wino_info_t* info);
+
+/* ------------------------------------------------ individual stages
+ * Enqueue one stage of the non-fused pipeline of an initialised blob on its
+ * stream, reading/writing the blob's workspace.  Layouts (fp16 precision):
+ *   U [t2][cp/8][kp][8]   V [t2][cp/8][pp][8]   M [t2][kp][pp]
+ * fp8 precisions use 32-byte groups: U [t2][cp/32][kp][32] (hi, then lo
+ * planes for FP8X2), V likewise with pp. */
+int wino_stage_filter(const void* param);
+int wino_stage_input(const void* param);
+int wino_stage_gemm(const void* param);
+int wino_stage_output(const void* param);
+
+/* ------------------------------------------------- device helpers
+ * Thin wrappers so that hosts without a HIP binding can move data. */
+int wino_device_malloc(void** ptr, size_t bytes);
+int wino_device_free(void* ptr);
+int wino_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int wino_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int wino_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

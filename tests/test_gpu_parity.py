@@ -1,0 +1,265 @@
+"""GPU parity of the HIP operator against the CPU oracle, through the C ABI.
+
+Stage tests compare each kernel with the oracle's stage model applied to the
+GPU's own inputs of that stage (tight, ulp-level).  End-to-end tests state
+the error against the FP64 direct convolution and against the reference's
+CPU direct convolution (verfiy.h semantics) for every configuration, with
+the tolerance written next to each case:
+
+  precision / algorithm        rel-L2 vs FP64 bound   (typical)
+  fp16  F(2x3), F(2x3) fused   2e-3                   (6e-4)
+  fp16  F(4x3)                 1e-2                   (4e-3)
+  fp8x2 F(2x3)                 5e-3                   (2e-3)
+  fp8x2 F(4x3) (4 products)    1.5e-2                 (9e-3)
+  fp8   F(2x3) / F(4x3)        0.15 / 0.6 (sanity; single e4m3 is reported, not a target)
+"""
+import numpy as np
+import pytest
+
+from conftest import rng_inputs
+
+pytestmark = pytest.mark.gpu
+
+F43, F23, F23F = 0, 1, 2
+FP16, FP8, FP8X2 = 0, 1, 2
+TOL = {(F23, FP16): 2e-3, (F23F, FP16): 2e-3, (F43, FP16): 1e-2,
+       (F23, FP8X2): 5e-3, (F43, FP8X2): 1.5e-2, (F23, FP8): 0.15, (F43, FP8): 0.6}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def ulp16(x):
+    a = np.abs(np.asarray(x, dtype=np.float64))
+    a16 = np.minimum(a, 65504.0).astype(np.float16)
+    return np.spacing(a16).astype(np.float64)
+
+
+def _mk(wc, gpu, x, wt, algo, prec=FP16):
+    n, c, h, w = x.shape
+    k = wt.shape[0]
+    dx, dw = gpu.to_dev(x), gpu.to_dev(wt)
+    dy = gpu.empty_f16((n, k, h, w))
+    op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k, algo=algo,
+                   precision=prec)
+    return op, (dx, dw, dy)
+
+
+def _u16(gpu, ptr, count):
+    return gpu.read_bytes(ptr, count * 2).view(np.float16).astype(np.float64)
+
+
+def _unblock(a, t2, groups, rows, g):
+    """[t2][groups][rows][g] -> [t2][rows][groups*g]"""
+    return a.reshape(t2, groups, rows, g).transpose(0, 2, 1, 3).reshape(t2, rows, groups * g)
+
+
+STAGE_SHAPES = [(2, 17, 9, 13, 27), (1, 64, 14, 14, 40), (3, 8, 7, 7, 5)]
+
+
+@pytest.mark.parametrize("algo", [F23, F43])
+@pytest.mark.parametrize("shape", STAGE_SHAPES)
+def test_stages_fp16(wc, gpu, oracle, algo, shape):
+    n, c, h, w, k = shape
+    say(f"stages fp16 algo={algo} shape={shape}")
+    x, wt = rng_inputs(100 + sum(shape), n, c, h, w, k, dist="clip4")
+    op, keep = _mk(wc, gpu, x, wt, algo)
+    try:
+        inf = op.info()
+        m, t2, Kp, Cp, Pp, P = inf.m, inf.t2, inf.kp, inf.cp, inf.pp, inf.tiles
+        # filter transform
+        op.stage("filter")
+        U = _unblock(_u16(gpu, inf.u, t2 * Cp * Kp), t2, Cp // 8, Kp, 8)
+        Uref = oracle.round_f16(oracle.wino_filter(m, wt))
+        d = np.abs(U[:, :k, :c] - Uref)
+        assert np.all(d <= ulp16(Uref)), d.max()
+        assert np.mean(d == 0) > 0.99
+        assert not U[:, k:, :].any() and not U[:, :, c:].any()  # pads are zero
+        # input transform
+        op.stage("input")
+        V = _unblock(_u16(gpu, inf.v, t2 * Cp * Pp), t2, Cp // 8, Pp, 8).transpose(0, 2, 1)
+        Vref = oracle.round_f16(oracle.wino_input(m, x))
+        d = np.abs(V[:, :c, :P] - Vref)
+        assert np.all(d <= ulp16(Vref)), d.max()
+        assert np.mean(d == 0) > 0.99
+        assert not V[:, c:, :].any() and not V[:, :, P:].any()
+        # batched GEMM on the GPU's own U and V
+        op.stage("gemm")
+        M = _u16(gpu, inf.mbuf, t2 * Kp * Pp).reshape(t2, Kp, Pp)
+        Mref = oracle.wino_gemm(U[:, :k, :c], V[:, :c, :P])
+        absdot = oracle.wino_gemm(np.abs(U[:, :k, :c]), np.abs(V[:, :c, :P]))
+        d = np.abs(M[:, :k, :P] - Mref)
+        assert np.all(d <= ulp16(Mref) + 2.0 ** -20 * absdot), (d - ulp16(Mref)).max()
+        # output transform of the GPU's own M
+        op.stage("output")
+        y = gpu.to_host(keep[2]).astype(np.float64)
+        Yref = oracle.wino_output(m, M[:, :k, :P], n, h, w)
+        Yabs = oracle.wino_output(m, np.abs(M[:, :k, :P]), n, h, w)  # coefficient-free bound
+        d = np.abs(y - oracle.round_f16(Yref))
+        assert np.all(d <= ulp16(Yref) + 2.0 ** -18 * np.abs(Yabs) * 64), d.max()
+    finally:
+        op.destroy()
+
+
+def _e2e(wc, gpu, oracle, x, wt, algo, prec, sample=None, label=""):
+    n, c, h, w = x.shape
+    k = wt.shape[0]
+    op, keep = _mk(wc, gpu, x, wt, algo, prec)
+    try:
+        op()
+        y = gpu.to_host(keep[2])
+        inf = op.info()
+    finally:
+        op.destroy()
+    assert np.isfinite(y).all()
+    if sample is None:
+        ref = oracle.conv2d_f64(x, wt).ravel()
+        ver = oracle.conv2d_verfiy(x, wt).ravel().astype(np.float64)
+        got = y.ravel().astype(np.float64)
+    else:
+        ref = oracle.conv2d_f64_sample(x, wt, sample)
+        ver = oracle.conv2d_verfiy_sample(x, wt, sample).astype(np.float64)
+        got = y.ravel()[sample].astype(np.float64)
+    e64 = oracle.errors(got, ref)
+    ever = oracle.errors(got, ver)
+    say(f"{label} algo={inf.algo} prec={inf.precision} shape={(n, c, h, w, k)} "
+        f"vsFP64 maxabs={e64[0]:.3e} relL2={e64[1]:.3e} | vs verfiy maxabs={ever[0]:.3e} "
+        f"relL2={ever[1]:.3e}")
+    return y, e64, ever, inf
+
+
+E2E_SHAPES = [
+    # (n, c, h, w, k, dist)
+    (1, 64, 56, 56, 64, "u11"),
+    (2, 3, 5, 5, 2, "u11"),          # tiny, C < 8
+    (3, 17, 7, 9, 27, "clip4"),      # ragged C, K; non-square
+    (2, 32, 14, 14, 64, "normal"),   # partial F(4x3) tiles (14 = 3*4 + 2)
+    (4, 40, 13, 6, 4, "clip4"),      # odd H, tiny K
+    (1, 128, 28, 28, 96, "clip4"),
+]
+
+
+@pytest.mark.parametrize("algo", [F23, F43, F23F])
+@pytest.mark.parametrize("shape", E2E_SHAPES)
+def test_e2e_fp16(wc, gpu, oracle, algo, shape):
+    n, c, h, w, k, dist = shape
+    x, wt = rng_inputs(7 + n + c + h, n, c, h, w, k, dist=dist)
+    say(f"e2e fp16 algo={algo} shape={shape}")
+    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, algo, FP16, label="e2e")
+    assert e64[1] <= TOL[(algo, FP16)]
+
+
+@pytest.mark.parametrize("algo", [F23, F43])
+@pytest.mark.parametrize("prec", [FP8X2, FP8])
+@pytest.mark.parametrize("shape", [(2, 64, 14, 14, 64, "normal"), (1, 130, 9, 11, 33, "clip4")])
+def test_e2e_fp8(wc, gpu, oracle, algo, prec, shape):
+    n, c, h, w, k, dist = shape
+    x, wt = rng_inputs(31 + c, n, c, h, w, k, dist=dist)
+    say(f"e2e fp8 algo={algo} prec={prec} shape={shape}")
+    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, algo, prec, label="e2e")
+    assert e64[1] <= TOL[(algo, prec)]
+
+
+@pytest.mark.parametrize("algo", [F23, F43])
+def test_fp8_stages(wc, gpu, oracle, algo):
+    """Quantizer, scale policy and hi/lo split of the fp8 transforms."""
+    n, c, h, w, k = 2, 130, 10, 10, 20
+    x, wt = rng_inputs(77, n, c, h, w, k, dist="clip4")
+    say(f"fp8 stages algo={algo}")
+    op, keep = _mk(wc, gpu, x, wt, algo, FP8X2)
+    try:
+        inf = op.info()
+        m, t2, Kp, Cp, Pp, P = inf.m, inf.t2, inf.kp, inf.cp, inf.pp, inf.tiles
+        op.stage("filter")
+        op.stage("input")
+        dec = np.array([oracle.e4m3_decode(b) for b in range(256)])
+        ub = gpu.read_bytes(inf.u, 2 * t2 * Cp * Kp).reshape(2, t2 * Cp * Kp)
+        us = gpu.read_bytes(inf.u_scale, t2 * Kp * 4).view(np.float32).reshape(t2, Kp)
+        vs = gpu.read_bytes(inf.v_scale, t2 * 4).view(np.float32)
+        hi = _unblock(dec[ub[0]], t2, Cp // 32, Kp, 32)[:, :k, :c]
+        lo = _unblock(dec[ub[1]], t2, Cp // 32, Kp, 32)[:, :k, :c]
+        Uref = oracle.wino_filter(m, wt)
+        s = us[:, :k].astype(np.float64)[:, :, None]
+        # scales: powers of two with the per-(t,k) amax landing in (224, 448]
+        assert np.all(np.log2(s) == np.round(np.log2(s)))
+        am = np.abs(Uref).max(axis=2) / s[:, :, 0]
+        assert np.all((am <= 448) & ((am > 224 * (1 - 1e-6)) | (am == 0)))
+        xs = Uref / s
+        assert np.all(np.abs(hi - xs) <= np.maximum(np.abs(xs) / 16, 2.0 ** -10) * (1 + 1e-6))
+        assert np.all(np.abs(hi + lo / 16 - xs) <= np.maximum(np.abs(xs) / 256, 2.0 ** -13) * 1.01)
+        # input: per-tile-position power-of-two scale from the input's amax and
+        # the worst-case row gains of B^T
+        gain = [2.0] * 4 if m == 2 else [10.0, 10.0, 10.0, 6.0, 6.0, 10.0]
+        amax = float(np.abs(x.astype(np.float64)).max())
+        T = m + 2
+        for t in range(t2):
+            bound = gain[t // T] * gain[t % T] * amax / 448.0
+            e = np.ceil(np.log2(bound))
+            exp_s = 2.0 ** e
+            assert vs[t] == exp_s, (t, vs[t], exp_s)
+        vb = gpu.read_bytes(inf.v, 2 * t2 * Cp * Pp).reshape(2, t2 * Cp * Pp)
+        vhi = _unblock(dec[vb[0]], t2, Cp // 32, Pp, 32).transpose(0, 2, 1)[:, :c, :P]
+        vlo = _unblock(dec[vb[1]], t2, Cp // 32, Pp, 32).transpose(0, 2, 1)[:, :c, :P]
+        Vs = oracle.wino_input(m, x) / vs.astype(np.float64)[:, None, None]
+        assert np.all(np.abs(vhi + vlo / 16 - Vs) <= np.maximum(np.abs(Vs) / 256, 2.0 ** -13) * 1.01)
+    finally:
+        op.destroy()
+
+
+def test_reference_entry_point_and_resident_filter(wc, gpu, oracle):
+    n, c, h, w, k = 2, 48, 12, 12, 32
+    x, wt = rng_inputs(55, n, c, h, w, k, dist="clip4")
+    x2, wt2 = rng_inputs(56, n, c, h, w, k, dist="clip4")
+    say("reference entry / resident filter")
+    for algo in (F23, F43, F23F):
+        dx, dw, dw2 = gpu.to_dev(x), gpu.to_dev(wt), gpu.to_dev(wt2)
+        dy, dy2 = gpu.empty_f16((n, k, h, w)), gpu.empty_f16((n, k, h, w))
+        op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k, algo=algo)
+        op.run_reference_entry()  # void conv2d_fp16(const void*)
+        y_ref = gpu.to_host(dy).copy()
+        op.destroy()
+        op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy2.data_ptr(), n, c, h, w, k, algo=algo,
+                       resident=True)
+        op()
+        op()
+        assert np.array_equal(gpu.to_host(dy2).view(np.uint16), y_ref.view(np.uint16))
+        # new weights through set_tensors invalidate the resident U
+        op.set_tensors(dx.data_ptr(), dw2.data_ptr(), dy2.data_ptr())
+        op()
+        y_new = gpu.to_host(dy2).astype(np.float64)
+        op.destroy()
+        e = oracle.errors(y_new, oracle.conv2d_f64(x, wt2))
+        assert e[1] <= TOL[(algo, FP16)], e
+
+
+def test_auto_selection_follows_reference(wc, gpu):
+    for (n, c, h, k, exp) in [(1, 64, 56, 64, F43), (64, 32, 28, 32, F23F), (1, 512, 7, 512, F23)]:
+        x, wt = rng_inputs(3, n, c, h, h, k)
+        y, inf = gpu.run_conv(wc, x, wt, algo=-1)
+        assert inf.algo == exp
+        assert np.isfinite(y).all()
+
+
+@pytest.mark.parametrize("algo,prec", [(F43, FP16), (F23F, FP16), (F43, FP8X2), (F23, FP16)])
+def test_baseline_config2_sampled(wc, gpu, oracle, algo, prec):
+    """N=64 C=256 28x28 K=256 (the throughput configuration), 1% sample."""
+    n, c, h, w, k = 64, 256, 28, 28, 256
+    x, wt = rng_inputs(2026, n, c, h, w, k, dist="clip4")
+    rng = np.random.default_rng(1)
+    sample = rng.choice(n * k * h * w, size=n * k * h * w // 100, replace=False)
+    say(f"config2 algo={algo} prec={prec}")
+    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, algo, prec, sample=sample, label="cfg2")
+    assert e64[1] <= TOL[(algo, prec)]
+
+
+@pytest.mark.parametrize("prec", [FP16, FP8X2, FP8])
+def test_baseline_config3_sampled(wc, gpu, oracle, prec):
+    """F(4x3), N=32 C=512 14x14 K=512, FP16 vs FP8 (1% sample)."""
+    n, c, h, w, k = 32, 512, 14, 14, 512
+    x, wt = rng_inputs(2027, n, c, h, w, k, dist="normal")
+    rng = np.random.default_rng(2)
+    sample = rng.choice(n * k * h * w, size=n * k * h * w // 100, replace=False)
+    say(f"config3 prec={prec}")
+    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, F43, prec, sample=sample, label="cfg3")
+    assert e64[1] <= TOL[(F43, prec)]
