@@ -86,14 +86,16 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
   pl->planes = precision == WINO_PREC_FP8X2 ? 2 : 1;
   if (algo == WINO_ALGO_F23_FUSED) {
     pl->gemm_cfg = -1;
-    pl->bk = 32;
-    pl->bp = 64;
-    pl->Kp = (uint32_t)round_up(pr->k, 32);
+    pl->bk = 64;  // output channels per fused block
+    pl->bp = 32;  // tiles per fused block
+    pl->Kp = (uint32_t)round_up(pr->k, 64);
     pl->Cp = (uint32_t)round_up(pr->c, 32);
     pl->Pp = pl->P;
     pl->u_bytes = (uint64_t)pl->t2 * pl->Cp * pl->Kp * 2;
     pl->u_off = 0;
-    pl->ws_bytes = align256(pl->u_bytes);
+    // 256 zero bytes after U: the fused kernel's direct-to-LDS staging reads
+    // padding from there
+    pl->ws_bytes = align256(pl->u_bytes) + 256;
     return WINO_OK;
   }
   pl->gemm_cfg = choose_gemm_cfg(pr->k, pl->P, pl->t2, precision);
@@ -314,8 +316,8 @@ int getkernelInfo(problem_t* problem, kernelInfo_t* kernelInfo, void* param) {
     memset(kernelInfo, 0, sizeof(*kernelInfo));
     const Plan& pl = prm->plan;
     if (pl.algo == WINO_ALGO_F23_FUSED) {
-      kernelInfo->blockx = (uint32_t)((pl.P + 63) / 64) * (pl.Kp / 32);
-      kernelInfo->threadx = 256;
+      kernelInfo->blockx = (uint32_t)((pl.P + pl.bp - 1) / pl.bp) * (pl.Kp / pl.bk);
+      kernelInfo->threadx = 512;
     } else {
       kernelInfo->blockx = (uint32_t)(pl.Pp / pl.bp);
       kernelInfo->blocky = pl.Kp / pl.bk;

@@ -111,15 +111,36 @@ __global__ __launch_bounds__(64) void k_filter_f8(const half_t* __restrict__ W,
 
 __global__ void k_amax_zero(float* amax) { *amax = 0.0f; }
 
+// max |x| over n halves: 16-byte loads when x is 16-byte aligned, a block
+// reduction in LDS and one atomic per block (non-negative floats order as
+// their bit patterns).
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_amax(const half_t* __restrict__ x, uint64_t n,
                                               float* __restrict__ amax) {
+  __shared__ float red[4];
   float a = 0.0f;
   const uint64_t stride = (uint64_t)gridDim.x * 256u;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride)
-    a = fmaxf(a, fabsf((float)x[i]));
+  const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  uint64_t tail = 0;
+  if (VEC) {
+    const uint64_t nv = n / 8;
+    const half8* xv = (const half8*)x;
+    for (uint64_t i = gid; i < nv; i += stride) {
+      const half8 v = xv[i];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a = fmaxf(a, fabsf((float)v[j]));
+    }
+    tail = nv * 8;
+  }
+  for (uint64_t i = tail + gid; i < n; i += stride) a = fmaxf(a, fabsf((float)x[i]));
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) a = fmaxf(a, __shfl_xor(a, o));
-  if ((threadIdx.x & 63u) == 0) atomicMax((unsigned int*)amax, __float_as_uint(a));
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    atomicMax((unsigned int*)amax, __float_as_uint(a));
+  }
 }
 
 // ------------------------------------------------------------------ input
@@ -282,10 +303,13 @@ hipError_t launch_filter_transform(const Plan& pl, const _Float16* w, uint32_t K
 
 hipError_t launch_input_amax(const _Float16* x, uint64_t count, float* amax, hipStream_t st) {
   hipLaunchKernelGGL(k_amax_zero, dim3(1), dim3(1), 0, st, amax);
-  uint32_t blocks = div_up(count, 256u * 8u);
-  if (blocks > 2048) blocks = 2048;
+  uint32_t blocks = div_up(count, 256u * 8u * 4u);
+  if (blocks > 1024) blocks = 1024;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_amax, dim3(blocks), dim3(256), 0, st, x, count, amax);
+  if (((uintptr_t)x & 15u) == 0)
+    hipLaunchKernelGGL(k_amax<true>, dim3(blocks), dim3(256), 0, st, x, count, amax);
+  else
+    hipLaunchKernelGGL(k_amax<false>, dim3(blocks), dim3(256), 0, st, x, count, amax);
   return hipGetLastError();
 }
 

@@ -74,7 +74,10 @@ def test_stages_fp16(wc, gpu, oracle, algo, shape):
         Uref = oracle.round_f16(oracle.wino_filter(m, wt))
         d = np.abs(U[:, :k, :c] - Uref)
         assert np.all(d <= ulp16(Uref)), d.max()
-        assert np.mean(d == 0) > 0.99
+        # F(2x3)'s G is dyadic (exact in FP32); F(4x3)'s 1/6, 1/12, 1/24 are
+        # not, so FP32 (with FMA contraction) lands on the other side of an
+        # fp16 rounding boundary for ~1-2% of entries -- never beyond 1 ulp.
+        assert np.mean(d == 0) > (0.99 if m == 2 else 0.96)
         assert not U[:, k:, :].any() and not U[:, :, c:].any()  # pads are zero
         # input transform
         op.stage("input")
