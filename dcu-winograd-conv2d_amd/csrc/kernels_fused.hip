@@ -123,31 +123,32 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
       }
     }
   };
+  // All 64 * MAXP units are written (the slot has room; units past the
+  // region hold zeros), so staging is branch-free.
   auto stage = [&]() {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       half_t* dst = xs + (4 * wave + i) * CH_H;
 #pragma unroll
       for (int j = 0; j < MAXP; ++j) {
-        if (lane + 64 * j < (int)g.units) {
-          if constexpr (EVENW)
-            ((uint32_t*)dst)[lane + 64 * j] = pf[i][j];
-          else
-            dst[lane + 64 * j] = __builtin_bit_cast(half_t, (uint16_t)pf[i][j]);
-        }
+        if constexpr (EVENW)
+          ((uint32_t*)dst)[lane + 64 * j] = pf[i][j];
+        else
+          dst[lane + 64 * j] = __builtin_bit_cast(half_t, (uint16_t)pf[i][j]);
       }
     }
   };
 
   // U fragments of this wave for one step: t = 4 xi + tl, k rows kh*32 + kt*16 + r16
-  half8 ua[4][2], un[4][2];
-  auto uload = [&](int s) {
+  typedef half8 ufrag_t[4][2];
+  half8 ua[4][2], ub[4][2];
+  auto uload = [&](int s, ufrag_t& dst) {
 #pragma unroll
     for (int tl = 0; tl < 4; ++tl)
 #pragma unroll
       for (int kt = 0; kt < 2; ++kt)
-        un[tl][kt] = U[((uint64_t)(4 * xi + tl) * g.C8p + 4 * s + grp) * g.Kp + kb * KB +
-                       kh * 32 + kt * 16 + r16];
+        dst[tl][kt] = U[((uint64_t)(4 * xi + tl) * g.C8p + 4 * s + grp) * g.Kp + kb * KB +
+                        kh * 32 + kt * 16 + r16];
   };
 
   // transform phase: thread -> (tile it, channel pair cp = 4 * cgrp + q);
@@ -159,13 +160,16 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
   const int ibi = it / rsz, ilr = (it % rsz) / (int)g.RW, ilc = it % (int)g.RW;
   const bool tvalid = it < (int)g.NB * rsz && n0 + ibi < g.N && th0 + ilr < g.TH &&
                       tw0 + ilc < g.TW;
-  // dword holding the patch's first half (the high half when W is even)
-  const int src0 = ((ibi * (int)g.LH + 2 * ilr) * (int)g.LWh) / 2 + ilc;
+  // Dword holding the patch's first half (the high half when W is even).
+  // A slot past the region's tiles transforms an arbitrary patch of its
+  // channel: its V values only reach MFMA output columns of tiles that are
+  // never stored, so the transform needs no branch.
+  const int src0 = tvalid ? ((ibi * (int)g.LH + 2 * ilr) * (int)g.LWh) / 2 + ilc : 0;
   const int lw2 = (int)g.LWh >> 1;
 
   auto transform = [&](half8* Vout) {
     half2_t vo[16];
-    if (tvalid) {
+    {
 #pragma unroll
       for (int cc = 0; cc < 2; ++cc) {
         const uint32_t* src = (const uint32_t*)(xs + (2 * cp + cc) * CH_H) + src0;
@@ -190,9 +194,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
 #pragma unroll
         for (int t = 0; t < 16; ++t) vo[t][cc] = (half_t)v[t];
       }
-    } else {
-#pragma unroll
-      for (int t = 0; t < 16; ++t) vo[t] = half2_t{(half_t)0.0f, (half_t)0.0f};
     }
     // V entry (t, 8-channel group cgrp, tile it), halves 2q .. 2q+1
     half2_t* V2 = (half2_t*)Vout;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
 #pragma unroll
       for (int c = 0; c < 2; ++c) acc[a][b][c] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  auto mfma_step = [&](const half8* Vin) {
+  auto mfma_step = [&](const half8* Vin, const ufrag_t& uf) {
 #pragma unroll
     for (int tl = 0; tl < 4; ++tl) {
       const int t = 4 * xi + tl;
@@ -220,36 +221,39 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
 #pragma unroll
         for (int pt = 0; pt < 2; ++pt)
           acc[tl][kt][pt] =
-              __builtin_amdgcn_mfma_f32_16x16x32_f16(ua[tl][kt], b[pt], acc[tl][kt][pt], 0, 0, 0);
+              __builtin_amdgcn_mfma_f32_16x16x32_f16(uf[tl][kt], b[pt], acc[tl][kt][pt], 0, 0, 0);
     }
   };
 
-  // Software pipeline over 32-channel steps.  In iteration s the MFMAs of
-  // step s (V buffer s&1, U registers ua) and the input transform of step
-  // s+1 (into V buffer (s+1)&1) are independent, so their instructions
-  // interleave and VALU and matrix cores work at the same time.
-  //   X(s+2), U(s+1) are loaded during iteration s.
+  // Software pipeline over 32-channel steps.  Iteration s runs the MFMAs of
+  // step s (V buffer s&1, U registers of step s) and the input transform of
+  // step s+1 (into V buffer (s+1)&1); they are independent and sit in one
+  // branch-free basic block, so their instructions interleave and the VALU
+  // and the matrix cores work at the same time.  X(s+2) and U(s+1) are
+  // loaded during iteration s.  The loop is unrolled by two so the U
+  // register sets alternate without copies; an odd step count runs one
+  // extra step whose channels are >= C (zeros, U of the last step).
   const int nsteps = (int)(g.C8p / 4);
   constexpr int VB = 16 * 4 * PB;  // half8 entries per V buffer
   prefetch(0);
-  uload(0);
+  uload(0, ua);
   stage();
   __syncthreads();
   prefetch(1);
   transform(Vl);
   __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
+  auto iter = [&](int s, const ufrag_t& ucur, ufrag_t& unext) {
     stage();  // X(s+1) -> xs (readers of xs finished before the last barrier)
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) ua[a][b] = un[a][b];
     __syncthreads();
-    uload(s + 1 < nsteps ? s + 1 : s);
+    uload(s + 1 < nsteps ? s + 1 : nsteps - 1, unext);
     prefetch(s + 2);
-    mfma_step(Vl + (s & 1) * VB);
-    if (s + 1 < nsteps) transform(Vl + ((s + 1) & 1) * VB);
+    mfma_step(Vl + (s & 1) * VB, ucur);
+    transform(Vl + ((s + 1) & 1) * VB);
     __syncthreads();
+  };
+  for (int s = 0; s < nsteps; s += 2) {
+    iter(s, ua, ub);
+    iter(s + 1, ub, ua);
   }
 
   // column pass of A^T M A in registers: R[j] = sum_nu A^T[j][nu] M[xi][nu]
