@@ -46,7 +46,7 @@ int validate(const problem_t* pr) {
 
 // GEMM tile choice: least padded work, then enough blocks to fill 256 CUs,
 // then the larger tile (more operand reuse).
-static int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision) {
+int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision) {
   int best = 0;
   double best_score = -1.0;
   for (int i = 0; i < gemm_cfg_count(); ++i) {

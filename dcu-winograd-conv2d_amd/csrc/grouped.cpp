@@ -1,0 +1,943 @@
+// grouped.cpp -- resident-filter engine, grouped batches and the
+// continuous-batching server (host code; kernels in kernels_*.hip).
+//
+// These have no counterpart in the reference, which ran one convolution per
+// call on the null stream and re-transformed its filter every time
+// (src/conv2d.cpp:118-133, src/winograd_4x3_non_fused.cpp:584).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "wino_internal.h"
+
+namespace wino {
+
+static uint64_t ru(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
+static uint64_t a256(uint64_t a) { return ru(a, 256); }
+static double now_us() {
+  return std::chrono::duration<double, std::micro>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+static int hipst(hipError_t e) {
+  if (e == hipSuccess) return WINO_OK;
+  fprintf(stderr, "winoconv: HIP error: %s\n", hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? WINO_ERR_ALLOC : WINO_ERR_LAUNCH;
+}
+
+// --------------------------------------------------------------- pool
+// First-fit allocator over one device allocation (256-byte granules).
+class Pool {
+ public:
+  ~Pool() {
+    if (base_) (void)hipFree(base_);
+  }
+  int init(size_t bytes) {
+    bytes = a256(bytes);
+    const hipError_t e = hipMalloc(&base_, bytes);
+    if (e != hipSuccess) {
+      fprintf(stderr, "winoconv: workspace pool of %zu bytes: %s\n", bytes, hipGetErrorString(e));
+      (void)hipGetLastError();
+      base_ = nullptr;
+      return WINO_ERR_ALLOC;
+    }
+    cap_ = bytes;
+    free_[0] = bytes;
+    return WINO_OK;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = a256(bytes ? bytes : 256);
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = a256(bytes ? bytes : 256);
+    std::lock_guard<std::mutex> lk(mu_);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  char* base() const { return (char*)base_; }
+  size_t capacity() const { return cap_; }
+  size_t used() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return used_;
+  }
+  size_t peak() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return peak_;
+  }
+
+ private:
+  void* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;
+  std::mutex mu_;
+};
+
+// ---------------------------------------------------------- tuning table
+struct TuneEntry {
+  uint32_t m, K, C;
+  uint64_t pmin, pmax;
+  int cfg, split;
+};
+
+}  // namespace wino
+
+using namespace wino;
+
+struct wino_engine {
+  int precision = WINO_PREC_FP16;
+  Pool pool;
+  std::vector<TuneEntry> tune;
+  struct Filter {
+    const void* weight = nullptr;
+    uint32_t K = 0, C = 0;
+    int algo = WINO_ALGO_AUTO;
+    void* U[2] = {nullptr, nullptr};  // [m==4]
+    float* us[2] = {nullptr, nullptr};
+    uint32_t Kp = 0, Cp = 0;
+    uint64_t u_bytes[2] = {0, 0};
+  };
+  // Reserved once (kMaxFilters) so that registration never moves a filter:
+  // the server thread reads filters while the caller registers more;
+  // `nfilters` publishes each one after it is complete.
+  static constexpr int kMaxFilters = 4096;
+  std::vector<Filter> filters;
+  std::atomic<int> nfilters{0};
+  std::mutex mu;
+  bool valid(int id) const { return id >= 0 && id < nfilters.load(std::memory_order_acquire); }
+};
+
+namespace wino {
+
+// Per-problem plan of a batched problem: the filter's U layout pads K to
+// 128 so that every tile height divides it; the GEMM covers only the row
+// blocks that hold real output channels.
+struct BPlan {
+  Plan pl;
+  uint32_t kblocks;   // GEMM row blocks computed
+  uint32_t split;
+  uint64_t v_off, m_off, m32_off, sc_off;
+  uint64_t footprint;
+  uint64_t items;     // GEMM work items (tiles x split)
+};
+
+// Work items one batch can hold (24 bytes each, staged in pinned memory).
+constexpr uint64_t kBatchMaxItems = (uint64_t)1 << 18;
+
+static void lookup_tile(const wino_engine* e, uint32_t m, uint32_t K, uint32_t C, uint64_t P,
+                        int t2, int* cfg, int* split, uint32_t nsteps) {
+  for (const TuneEntry& te : e->tune)
+    if (te.m == m && te.K == K && te.C == C && P >= te.pmin && P <= te.pmax &&
+        te.cfg >= 0 && te.cfg < gemm_cfg_count()) {
+      *cfg = te.cfg;
+      *split = std::max(1, std::min<int>(te.split, (int)nsteps));
+      return;
+    }
+  *cfg = choose_gemm_cfg(K, P, t2, e->precision);
+  const GemmCfg gc = gemm_cfg(*cfg);
+  const uint64_t tiles = (uint64_t)((K + gc.bk - 1) / gc.bk) * ((P + gc.bp - 1) / gc.bp) * t2;
+  // a problem too small to fill the device on its own splits its reduction
+  int sp = 1;
+  while (tiles * sp < 256 && nsteps >= (uint32_t)(8 * sp) && sp < 4) sp *= 2;
+  *split = sp;
+}
+
+static int plan_batched(const wino_engine* e, uint32_t K, uint32_t C, uint32_t n, uint32_t h,
+                        uint32_t w, int m, int cfg, int split, BPlan* bp) {
+  memset(bp, 0, sizeof(*bp));
+  if (m != 2 && m != 4) return WINO_ERR_UNSUPPORTED;
+  Plan& pl = bp->pl;
+  pl.algo = m == 4 ? WINO_ALGO_F43 : WINO_ALGO_F23;
+  pl.precision = e->precision;
+  pl.m = m;
+  pl.T = m + 2;
+  pl.t2 = pl.T * pl.T;
+  pl.oh = h;
+  pl.ow = w;
+  pl.tiles_h = (h + m - 1) / m;
+  pl.tiles_w = (w + m - 1) / m;
+  pl.P = (uint64_t)n * pl.tiles_h * pl.tiles_w;
+  pl.planes = e->precision == WINO_PREC_FP8X2 ? 2 : 1;
+  pl.Cp = (uint32_t)ru(C, e->precision == WINO_PREC_FP16 ? 32 : 128);
+  pl.Kp = (uint32_t)ru(K, 128);
+  const uint32_t nsteps = e->precision == WINO_PREC_FP16 ? pl.Cp / 32 : pl.Cp / 128;
+  if (cfg < 0) lookup_tile(e, m, K, C, pl.P, pl.t2, &cfg, &split, nsteps);
+  if (cfg >= gemm_cfg_count() || split < 1) return WINO_ERR_INVALID;
+  split = std::min<int>(split, (int)nsteps);
+  const GemmCfg gc = gemm_cfg(cfg);
+  pl.gemm_cfg = cfg;
+  pl.bk = gc.bk;
+  pl.bp = gc.bp;
+  pl.Pp = ru(pl.P, gc.bp);
+  bp->kblocks = (K + gc.bk - 1) / gc.bk;
+  bp->split = (uint32_t)split;
+  const uint64_t vb = e->precision == WINO_PREC_FP16
+                          ? (uint64_t)pl.t2 * pl.Cp * pl.Pp * 2
+                          : (uint64_t)pl.planes * pl.t2 * pl.Cp * pl.Pp;
+  const uint64_t mb = split > 1 ? 0 : (uint64_t)pl.t2 * pl.Kp * pl.Pp * 2;
+  const uint64_t m32 = split > 1 ? (uint64_t)split * pl.t2 * pl.Kp * pl.Pp * 4 : 0;
+  const uint64_t sc = e->precision == WINO_PREC_FP16 ? 0 : 512;  // v_scale[t2] + amax
+  uint64_t off = 0;
+  bp->v_off = off;
+  off += a256(vb);
+  bp->m_off = off;
+  off += a256(mb);
+  bp->m32_off = off;
+  off += a256(m32);
+  bp->sc_off = off;
+  off += a256(sc);
+  bp->footprint = off;
+  bp->items = (uint64_t)pl.t2 * bp->kblocks * (pl.Pp / gc.bp) * (uint64_t)split;
+  pl.v_bytes = vb;
+  pl.m_bytes = mb ? mb : m32;
+  pl.ws_bytes = off;
+  return WINO_OK;
+}
+
+static int filter_m(const wino_engine::Filter& f, int algo, uint32_t n, uint32_t h, uint32_t w) {
+  int a = algo == WINO_ALGO_AUTO ? f.algo : algo;
+  if (a == WINO_ALGO_AUTO) {
+    problem_t pr;
+    memset(&pr, 0, sizeof(pr));
+    pr.n = n;
+    pr.c = f.C;
+    pr.h = h;
+    pr.w = w;
+    pr.k = f.K;
+    pr.r = pr.s = 3;
+    pr.u = pr.v = 1;
+    pr.p = pr.q = 1;
+    a = reference_select(&pr);
+  }
+  if (a == WINO_ALGO_F43) return 4;
+  if (a == WINO_ALGO_F23 || a == WINO_ALGO_F23_FUSED) return 2;
+  return 0;
+}
+
+}  // namespace wino
+
+struct wino_batch {
+  wino_engine* eng = nullptr;
+  int max_probs = 0;
+  uint64_t max_items = 0;
+  struct Entry {
+    int filter;
+    BPlan bp;
+    uint64_t pool_off;
+    bool has_ws;
+    GProb gp;
+  };
+  std::vector<Entry> entries;
+  // device descriptors: header | probs | items (one contiguous allocation)
+  char* dev = nullptr;
+  char* host = nullptr;  // pinned staging of the same layout
+  size_t probs_off = 0, items_off = 0, bytes = 0;
+  uint32_t grid_in = 0, grid_gemm = 0, grid_out = 0, grid_amax = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+namespace wino {
+
+static int batch_enqueue_kernels(wino_batch* b, hipStream_t st) {
+  const GroupHeader* hdr = (const GroupHeader*)b->dev;
+  const GProb* probs = (const GProb*)(b->dev + b->probs_off);
+  const GItem* items = (const GItem*)(b->dev + b->items_off);
+  const int prec = b->eng->precision;
+  hipError_t e = hipSuccess;
+  if (prec != WINO_PREC_FP16) e = launch_amax_grouped(hdr, probs, b->grid_amax, st);
+  if (e == hipSuccess) e = launch_input_grouped(prec, hdr, probs, b->grid_in, st);
+  if (e == hipSuccess) e = launch_gemm_grouped(prec, hdr, probs, items, b->grid_gemm, st);
+  if (e == hipSuccess) e = launch_output_grouped(hdr, probs, b->grid_out, st);
+  return hipst(e);
+}
+
+}  // namespace wino
+
+extern "C" {
+
+// ================================================================= engine
+
+int wino_engine_create(wino_engine** engine, int precision, size_t workspace_bytes) {
+  if (!engine || precision < WINO_PREC_FP16 || precision > WINO_PREC_FP8X2) return WINO_ERR_INVALID;
+  wino_engine* e = new wino_engine();
+  e->precision = precision;
+  e->filters.reserve(wino_engine::kMaxFilters);
+  int st = e->pool.init(workspace_bytes);
+  if (st) {
+    delete e;
+    return st;
+  }
+  st = hipst(grouped_gemm_prepare(precision));
+  if (st) {
+    delete e;
+    return st;
+  }
+  *engine = e;
+  return WINO_OK;
+}
+
+int wino_engine_destroy(wino_engine* e) {
+  if (!e) return WINO_ERR_INVALID;
+  (void)hipDeviceSynchronize();
+  for (auto& f : e->filters)
+    for (int i = 0; i < 2; ++i)
+      if (f.U[i]) (void)hipFree(f.U[i]);
+  delete e;
+  return WINO_OK;
+}
+
+int wino_engine_load_tuning(wino_engine* e, const char* path) {
+  if (!e || !path) return WINO_ERR_INVALID;
+  FILE* fp = fopen(path, "r");
+  if (!fp) return WINO_ERR_NOT_FOUND;
+  char line[512];
+  std::vector<TuneEntry> tab;
+  while (fgets(line, sizeof line, fp)) {
+    char* hash = strchr(line, '#');
+    if (hash) *hash = 0;
+    TuneEntry t;
+    unsigned long long pmin, pmax;
+    if (sscanf(line, "%u %u %u %llu %llu %d %d", &t.m, &t.K, &t.C, &pmin, &pmax, &t.cfg,
+               &t.split) == 7) {
+      t.pmin = pmin;
+      t.pmax = pmax;
+      tab.push_back(t);
+    }
+  }
+  fclose(fp);
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->tune = tab;
+  return WINO_OK;
+}
+
+static int transform_filter(wino_engine* e, wino_engine::Filter& f) {
+  for (int i = 0; i < 2; ++i) {
+    if (!f.U[i]) continue;
+    Plan pl;
+    memset(&pl, 0, sizeof(pl));
+    pl.precision = e->precision;
+    pl.m = i ? 4 : 2;
+    pl.T = pl.m + 2;
+    pl.t2 = pl.T * pl.T;
+    pl.Kp = f.Kp;
+    pl.Cp = f.Cp;
+    pl.planes = e->precision == WINO_PREC_FP8X2 ? 2 : 1;
+    int st = hipst(launch_filter_transform(pl, (const _Float16*)f.weight, f.K, f.C, f.U[i],
+                                           f.us[i], nullptr));
+    if (st) return st;
+  }
+  return hipst(hipDeviceSynchronize());
+}
+
+int wino_engine_register_filter(wino_engine* e, const void* weight, uint32_t k, uint32_t c,
+                                int algo, int* filter_id) {
+  if (!e || !weight || !filter_id || !k || !c) return WINO_ERR_INVALID;
+  if (algo != WINO_ALGO_AUTO && algo != WINO_ALGO_F23 && algo != WINO_ALGO_F43 &&
+      algo != WINO_ALGO_F23_FUSED)
+    return WINO_ERR_INVALID;
+  wino_engine::Filter f;
+  f.weight = weight;
+  f.K = k;
+  f.C = c;
+  f.algo = algo;
+  f.Kp = (uint32_t)ru(k, 128);
+  f.Cp = (uint32_t)ru(c, e->precision == WINO_PREC_FP16 ? 32 : 128);
+  const bool need2 = algo != WINO_ALGO_F43, need4 = algo == WINO_ALGO_F43 || algo == WINO_ALGO_AUTO;
+  for (int i = 0; i < 2; ++i) {
+    if (!(i ? need4 : need2)) continue;
+    const int t2 = i ? 36 : 16;
+    const uint64_t ub = e->precision == WINO_PREC_FP16
+                            ? (uint64_t)t2 * f.Cp * f.Kp * 2
+                            : (uint64_t)(e->precision == WINO_PREC_FP8X2 ? 2 : 1) * t2 * f.Cp * f.Kp;
+    const uint64_t sb = e->precision == WINO_PREC_FP16 ? 0 : (uint64_t)t2 * f.Kp * 4;
+    void* mem = nullptr;
+    if (hipMalloc(&mem, a256(ub) + a256(sb)) != hipSuccess) {
+      (void)hipGetLastError();
+      for (int j = 0; j < 2; ++j)
+        if (f.U[j]) (void)hipFree(f.U[j]);
+      return WINO_ERR_ALLOC;
+    }
+    (void)hipMemset(mem, 0, a256(ub) + a256(sb));  // pads read as zero
+    f.U[i] = mem;
+    f.us[i] = sb ? (float*)((char*)mem + a256(ub)) : nullptr;
+    f.u_bytes[i] = ub;
+  }
+  int st = transform_filter(e, f);
+  if (st) {
+    for (int j = 0; j < 2; ++j)
+      if (f.U[j]) (void)hipFree(f.U[j]);
+    return st;
+  }
+  std::lock_guard<std::mutex> lk(e->mu);
+  if ((int)e->filters.size() >= wino_engine::kMaxFilters) {
+    for (int j = 0; j < 2; ++j)
+      if (f.U[j]) (void)hipFree(f.U[j]);
+    return WINO_ERR_CAPACITY;
+  }
+  e->filters.push_back(f);
+  *filter_id = (int)e->filters.size() - 1;
+  e->nfilters.store((int)e->filters.size(), std::memory_order_release);
+  return WINO_OK;
+}
+
+int wino_engine_refresh_filter(wino_engine* e, int id) {
+  if (!e || !e->valid(id)) return WINO_ERR_NOT_FOUND;
+  return transform_filter(e, e->filters[id]);
+}
+
+int wino_engine_footprint(wino_engine* e, int id, uint32_t n, uint32_t h, uint32_t w,
+                          size_t* bytes) {
+  if (!e || !bytes || !n || !h || !w) return WINO_ERR_INVALID;
+  if (!e->valid(id)) return WINO_ERR_NOT_FOUND;
+  const auto& f = e->filters[id];
+  const int m = filter_m(f, WINO_ALGO_AUTO, n, h, w);
+  BPlan bp;
+  int st = plan_batched(e, f.K, f.C, n, h, w, m, -1, 1, &bp);
+  if (st) return st;
+  *bytes = bp.footprint;
+  return WINO_OK;
+}
+
+int wino_engine_pool_usage(wino_engine* e, size_t* used, size_t* capacity) {
+  if (!e) return WINO_ERR_INVALID;
+  if (used) *used = e->pool.used();
+  if (capacity) *capacity = e->pool.capacity();
+  return WINO_OK;
+}
+
+// ================================================================= batch
+
+int wino_batch_create(wino_engine* e, int max_problems, wino_batch** out) {
+  if (!e || !out || max_problems <= 0) return WINO_ERR_INVALID;
+  wino_batch* b = new wino_batch();
+  b->eng = e;
+  b->max_probs = max_problems;
+  b->max_items = kBatchMaxItems;
+  b->probs_off = 256;
+  b->items_off = a256(b->probs_off + sizeof(GProb) * (size_t)max_problems);
+  b->bytes = b->items_off + sizeof(GItem) * b->max_items;
+  if (hipMalloc((void**)&b->dev, b->bytes) != hipSuccess ||
+      hipHostMalloc((void**)&b->host, b->bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    if (b->dev) (void)hipFree(b->dev);
+    delete b;
+    return WINO_ERR_ALLOC;
+  }
+  memset(b->host, 0, b->bytes);
+  // fixed grids: virtual-block loops cover any batch
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (cus <= 0) cus = 256;
+  b->grid_gemm = (uint32_t)cus * 2;
+  b->grid_in = (uint32_t)cus * 4;
+  b->grid_out = (uint32_t)cus * 8;
+  b->grid_amax = (uint32_t)cus * 2;
+  (void)hipGetLastError();
+  *out = b;
+  return WINO_OK;
+}
+
+int wino_batch_reset(wino_batch* b) {
+  if (!b) return WINO_ERR_INVALID;
+  for (auto& en : b->entries)
+    if (en.has_ws) b->eng->pool.release(en.pool_off, en.bp.footprint);
+  b->entries.clear();
+  return WINO_OK;
+}
+
+int wino_batch_destroy(wino_batch* b) {
+  if (!b) return WINO_ERR_INVALID;
+  (void)hipDeviceSynchronize();
+  wino_batch_reset(b);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->dev) (void)hipFree(b->dev);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+  return WINO_OK;
+}
+
+int wino_batch_size(wino_batch* b) { return b ? (int)b->entries.size() : WINO_ERR_INVALID; }
+
+static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, void* out,
+                       uint32_t n, uint32_t h, uint32_t w) {
+  wino_engine* e = b->eng;
+  const auto& f = e->filters[en.filter];
+  const Plan& pl = en.bp.pl;
+  char* ws = e->pool.base() + en.pool_off;
+  GProb& g = en.gp;
+  memset(&g, 0, sizeof(g));
+  const int mi = pl.m == 4 ? 1 : 0;
+  g.x = (const _Float16*)in;
+  g.y = (_Float16*)out;
+  g.U = f.U[mi];
+  g.us = f.us[mi];
+  g.V = ws + en.bp.v_off;
+  g.M = en.bp.split > 1 ? nullptr : (_Float16*)(ws + en.bp.m_off);
+  g.M32 = en.bp.split > 1 ? (float*)(ws + en.bp.m32_off) : nullptr;
+  g.vs = e->precision == WINO_PREC_FP16 ? nullptr : (float*)(ws + en.bp.sc_off);
+  g.amax = e->precision == WINO_PREC_FP16 ? nullptr : (float*)(ws + en.bp.sc_off + 256);
+  g.ig = input_geom(pl, n, f.C, h, w, 40u * 1024u);
+  g.P = pl.P;
+  g.Pp = pl.Pp;
+  g.K = f.K;
+  g.Kp = pl.Kp;
+  g.Cp = pl.Cp;
+  g.m = (uint32_t)pl.m;
+  g.t2 = (uint32_t)pl.t2;
+  g.OH = h;
+  g.OW = w;
+  g.cfg = (uint32_t)pl.gemm_cfg;
+  g.split = en.bp.split;
+  g.nsteps = e->precision == WINO_PREC_FP16 ? pl.Cp / 32 : pl.Cp / 128;
+  // Pool memory is reused between problems, so the input transform also
+  // writes the zero channel groups of the padded reduction (c >= C); the
+  // padded tile columns (p >= P) only reach GEMM columns nobody reads.
+  g.in_gx = n * g.ig.nwr * g.ig.nwc;
+  g.in_count = g.in_gx * g.ig.CGp;
+  g.out_count = (uint32_t)(((uint64_t)f.K * ((pl.P + 1) / 2) + 255) / 256);
+  const uint64_t nin = (uint64_t)n * f.C * h * w;
+  g.amax_count = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, nin / (256 * 64)));
+}
+
+int wino_batch_add(wino_batch* b, int filter_id, const void* in, void* out, uint32_t n,
+                   uint32_t h, uint32_t w, int algo, int* index) {
+  if (!b || !in || !out || !n || !h || !w) return WINO_ERR_INVALID;
+  wino_engine* e = b->eng;
+  if (!e->valid(filter_id)) return WINO_ERR_NOT_FOUND;
+  if ((int)b->entries.size() >= b->max_probs) return WINO_ERR_CAPACITY;
+  const auto& f = e->filters[filter_id];
+  const int m = filter_m(f, algo, n, h, w);
+  if (!m || !f.U[m == 4 ? 1 : 0]) return WINO_ERR_UNSUPPORTED;
+  if ((uint64_t)n * f.C * h * w >= (1ull << 31) || (uint64_t)n * f.K * h * w >= (1ull << 31))
+    return WINO_ERR_UNSUPPORTED;
+  wino_batch::Entry en;
+  en.filter = filter_id;
+  int st = plan_batched(e, f.K, f.C, n, h, w, m, -1, 1, &en.bp);
+  if (st) return st;
+  if (en.bp.footprint > e->pool.capacity() || en.bp.items > b->max_items)
+    return WINO_ERR_UNSUPPORTED;  // can never fit, whatever else is running
+  uint64_t items = en.bp.items;
+  for (const auto& o : b->entries) items += o.bp.items;
+  if (items > b->max_items) return WINO_ERR_CAPACITY;
+  if (!e->pool.alloc(en.bp.footprint, &en.pool_off)) return WINO_ERR_CAPACITY;
+  en.has_ws = true;
+  fill_gprob(b, en, in, out, n, h, w);
+  b->entries.push_back(en);
+  if (index) *index = (int)b->entries.size() - 1;
+  return WINO_OK;
+}
+
+int wino_batch_set_tile(wino_batch* b, int index, int cfg, int split) {
+  if (!b || index < 0 || index >= (int)b->entries.size()) return WINO_ERR_NOT_FOUND;
+  auto& en = b->entries[index];
+  wino_engine* e = b->eng;
+  const auto& f = e->filters[en.filter];
+  const GProb old = en.gp;
+  const uint32_t n = old.ig.N, h = old.OH, w = old.OW;
+  BPlan bp;
+  int st = plan_batched(e, f.K, f.C, n, h, w, (int)old.m, cfg, split, &bp);
+  if (st) return st;
+  uint64_t items = bp.items;
+  for (const auto& o : b->entries) items += &o == &en ? 0 : o.bp.items;
+  if (items > b->max_items) return WINO_ERR_CAPACITY;
+  size_t off;
+  if (!e->pool.alloc(bp.footprint, &off)) return WINO_ERR_CAPACITY;
+  e->pool.release(en.pool_off, en.bp.footprint);
+  en.bp = bp;
+  en.pool_off = off;
+  fill_gprob(b, en, old.x, old.y, n, h, w);
+  return WINO_OK;
+}
+
+int wino_batch_info(wino_batch* b, int index, wino_info_t* info) {
+  if (!b || !info || index < 0 || index >= (int)b->entries.size()) return WINO_ERR_NOT_FOUND;
+  const auto& en = b->entries[index];
+  const Plan& pl = en.bp.pl;
+  memset(info, 0, sizeof(*info));
+  info->algo = pl.algo;
+  info->precision = pl.precision;
+  info->m = pl.m;
+  info->t2 = pl.t2;
+  info->oh = pl.oh;
+  info->ow = pl.ow;
+  info->tiles_h = pl.tiles_h;
+  info->tiles_w = pl.tiles_w;
+  info->tiles = pl.P;
+  info->kp = pl.Kp;
+  info->cp = pl.Cp;
+  info->pp = pl.Pp;
+  info->gemm_bk = pl.bk;
+  info->gemm_bp = pl.bp;
+  info->ws_bytes = en.bp.footprint;
+  info->v_bytes = pl.v_bytes;
+  info->m_bytes = pl.m_bytes;
+  info->u = en.gp.U;
+  info->v = en.gp.V;
+  info->mbuf = en.gp.M ? (void*)en.gp.M : (void*)en.gp.M32;
+  info->u_scale = en.gp.us;
+  info->v_scale = en.gp.vs;
+  // the reduction split is reported in scale_bytes' place for batched problems
+  info->scale_bytes = en.bp.split;
+  return WINO_OK;
+}
+
+// Writes header, problem descriptors and LPT-ordered work items into the
+// pinned staging buffer; returns its used size.
+static size_t build_descriptors(wino_batch* b) {
+  GroupHeader* hdr = (GroupHeader*)b->host;
+  GProb* probs = (GProb*)(b->host + b->probs_off);
+  GItem* items = (GItem*)(b->host + b->items_off);
+  memset(hdr, 0, sizeof(*hdr));
+  uint32_t in_first = 0, out_first = 0, amax_first = 0;
+  struct Work {
+    uint64_t cost;
+    GItem it;
+  };
+  std::vector<Work> work;
+  for (size_t i = 0; i < b->entries.size(); ++i) {
+    GProb g = b->entries[i].gp;
+    const BPlan& bp = b->entries[i].bp;
+    g.in_first = in_first;
+    in_first += g.in_count;
+    g.out_first = out_first;
+    out_first += g.out_count;
+    g.amax_first = amax_first;
+    amax_first += g.amax_count;
+    probs[i] = g;
+    const uint64_t pblocks = bp.pl.Pp / bp.pl.bp;
+    const uint64_t cost = (uint64_t)bp.pl.bk * bp.pl.bp * (g.nsteps / bp.split);
+    for (uint32_t t = 0; t < g.t2; ++t)
+      for (uint32_t kb = 0; kb < bp.kblocks; ++kb)
+        for (uint64_t pb = 0; pb < pblocks; ++pb)
+          for (uint32_t sp = 0; sp < bp.split; ++sp) {
+            Work wk;
+            wk.cost = cost;
+            wk.it.prob = (uint32_t)i;
+            wk.it.t = t;
+            wk.it.kb = kb;
+            wk.it.sp = sp;
+            wk.it.pb = pb;
+            work.push_back(wk);
+          }
+  }
+  std::stable_sort(work.begin(), work.end(),
+                   [](const Work& a, const Work& c) { return a.cost > c.cost; });
+  const size_t nitems = work.size();  // <= max_items: checked as problems are added
+  for (size_t i = 0; i < nitems; ++i) items[i] = work[i].it;
+  hdr->nprob = (uint32_t)b->entries.size();
+  hdr->in_blocks = in_first;
+  hdr->out_blocks = out_first;
+  hdr->nitems = (uint32_t)nitems;
+  hdr->amax_blocks = amax_first;
+  return b->items_off + nitems * sizeof(GItem);
+}
+
+int wino_batch_launch(wino_batch* b, void* stream) {
+  if (!b) return WINO_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t used = build_descriptors(b);
+  int rc = hipst(hipMemcpyAsync(b->dev, b->host, used, hipMemcpyHostToDevice, st));
+  if (rc) return rc;
+  if (b->exec) return hipst(hipGraphLaunch(b->exec, st));
+  return batch_enqueue_kernels(b, st);
+}
+
+int wino_batch_capture(wino_batch* b, void* stream) {
+  if (!b) return WINO_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  if (!st) return WINO_ERR_INVALID;  // the null stream cannot be captured
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  int rc = hipst(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  if (rc) return rc;
+  rc = batch_enqueue_kernels(b, st);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hipst(e);
+  b->graph = g;
+  return hipst(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+}
+
+}  // extern "C"
+
+// ================================================================= server
+
+struct wino_server {
+  wino_engine* eng = nullptr;
+  int max_batch = 16;
+  bool use_graph = false;
+  int device = 0;
+  struct Req {
+    int filter;
+    const void* in;
+    void* out;
+    uint32_t n, h, w;
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int status = WINO_OK;
+  };
+  struct Slot {
+    wino_batch* batch = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t event = nullptr;
+    bool busy = false;
+    bool captured = false;
+    std::vector<uint64_t> reqs;
+  };
+  std::vector<Slot> slots;
+  std::deque<uint64_t> pending;
+  std::unordered_map<uint64_t, Req> reqs;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::thread th;
+  std::atomic<bool> stop{false};
+  uint64_t next_id = 1;
+  wino_server_stats_t stats;
+  int error = WINO_OK;
+};
+
+static void server_loop(wino_server* s) {
+  (void)hipSetDevice(s->device);
+  while (true) {
+    bool progressed = false;
+    uint32_t inflight = 0;
+    // retire finished batches
+    for (auto& sl : s->slots) {
+      if (!sl.busy) continue;
+      if (hipEventQuery(sl.event) != hipSuccess) {
+        (void)hipGetLastError();
+        ++inflight;
+        continue;
+      }
+      const double t = now_us();
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (uint64_t id : sl.reqs) {
+          auto it = s->reqs.find(id);
+          if (it != s->reqs.end()) {
+            it->second.done = true;
+            it->second.t_done = t;
+          }
+          ++s->stats.completed;
+        }
+      }
+      sl.reqs.clear();
+      wino_batch_reset(sl.batch);
+      sl.busy = false;
+      progressed = true;
+      s->cv.notify_all();
+    }
+    // admit pending requests into free slots (FIFO; stop at the first one
+    // the workspace pool cannot hold yet)
+    for (auto& sl : s->slots) {
+      if (sl.busy) continue;
+      std::vector<uint64_t> taken;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        while (!s->pending.empty() && (int)taken.size() < s->max_batch) {
+          const uint64_t id = s->pending.front();
+          const auto& r = s->reqs[id];
+          const int st = wino_batch_add(sl.batch, r.filter, r.in, r.out, r.n, r.h, r.w,
+                                        WINO_ALGO_AUTO, nullptr);
+          if (st == WINO_ERR_CAPACITY) break;
+          s->pending.pop_front();
+          if (st) {  // cannot run: complete it with its error
+            s->reqs[id].done = true;
+            s->reqs[id].status = st;
+            s->reqs[id].t_done = now_us();
+            ++s->stats.rejected;
+            ++s->stats.completed;
+            continue;
+          }
+          taken.push_back(id);
+        }
+      }
+      s->cv.notify_all();  // requests completed with an error wake drain()
+      if (taken.empty()) continue;
+      int st = WINO_OK;
+      if (s->use_graph && !sl.captured) {
+        st = wino_batch_capture(sl.batch, sl.stream);
+        sl.captured = st == WINO_OK;
+      }
+      if (!st) st = wino_batch_launch(sl.batch, sl.stream);
+      if (!st) st = hipst(hipEventRecord(sl.event, sl.stream));
+      if (st) s->error = st;
+      sl.reqs = taken;
+      sl.busy = true;
+      ++inflight;
+      progressed = true;
+      std::lock_guard<std::mutex> lk(s->mu);
+      ++s->stats.batches;
+      s->stats.max_batch_problems = std::max<uint32_t>(s->stats.max_batch_problems, (uint32_t)taken.size());
+      s->stats.max_inflight_batches = std::max<uint32_t>(s->stats.max_inflight_batches, inflight);
+      s->stats.pool_peak = std::max<size_t>(s->stats.pool_peak, s->eng->pool.used());
+    }
+    if (s->stop.load() && inflight == 0) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->pending.empty()) break;
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (inflight)
+        s->cv.wait_for(lk, std::chrono::microseconds(20));
+      else
+        s->cv.wait_for(lk, std::chrono::milliseconds(1),
+                       [&] { return !s->pending.empty() || s->stop.load(); });
+    }
+  }
+}
+
+extern "C" {
+
+int wino_server_create(wino_engine* e, int slots, int max_batch, int use_graph,
+                       wino_server** out) {
+  if (!e || !out || slots <= 0 || slots > 8 || max_batch <= 0) return WINO_ERR_INVALID;
+  wino_server* s = new wino_server();
+  s->eng = e;
+  s->max_batch = max_batch;
+  s->use_graph = use_graph != 0;
+  memset(&s->stats, 0, sizeof(s->stats));
+  (void)hipGetDevice(&s->device);
+  s->slots.resize(slots);
+  for (auto& sl : s->slots) {
+    int st = wino_batch_create(e, max_batch, &sl.batch);
+    if (!st) st = hipst(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+    if (!st) st = hipst(hipEventCreateWithFlags(&sl.event, hipEventDisableTiming));
+    if (st) {
+      for (auto& q : s->slots) {
+        if (q.batch) wino_batch_destroy(q.batch);
+        if (q.stream) (void)hipStreamDestroy(q.stream);
+        if (q.event) (void)hipEventDestroy(q.event);
+      }
+      delete s;
+      return st;
+    }
+  }
+  s->th = std::thread(server_loop, s);
+  *out = s;
+  return WINO_OK;
+}
+
+int wino_server_submit(wino_server* s, int filter_id, const void* in, void* out, uint32_t n,
+                       uint32_t h, uint32_t w, uint64_t* request_id) {
+  if (!s || !in || !out || !request_id) return WINO_ERR_INVALID;
+  wino_engine* e = s->eng;
+  if (!e->valid(filter_id)) return WINO_ERR_NOT_FOUND;
+  if (!n || !h || !w) return WINO_ERR_INVALID;
+  const auto& f = e->filters[filter_id];
+  const int m = filter_m(f, WINO_ALGO_AUTO, n, h, w);
+  if (!m || !f.U[m == 4 ? 1 : 0]) return WINO_ERR_UNSUPPORTED;
+  BPlan bp;
+  int st = plan_batched(e, f.K, f.C, n, h, w, m, -1, 1, &bp);
+  if (st) return st;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (bp.footprint > e->pool.capacity() || bp.items > kBatchMaxItems) {
+    ++s->stats.rejected;
+    return WINO_ERR_CAPACITY;
+  }
+  wino_server::Req r;
+  r.filter = filter_id;
+  r.in = in;
+  r.out = out;
+  r.n = n;
+  r.h = h;
+  r.w = w;
+  r.t_submit = now_us();
+  const uint64_t id = s->next_id++;
+  s->reqs[id] = r;
+  s->pending.push_back(id);
+  ++s->stats.submitted;
+  *request_id = id;
+  s->cv.notify_all();
+  return WINO_OK;
+}
+
+int wino_server_poll(wino_server* s, uint64_t id, double* latency_us) {
+  if (!s) return WINO_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return WINO_ERR_NOT_FOUND;
+  if (!it->second.done) return WINO_ERR_BUSY;
+  if (latency_us) *latency_us = it->second.t_done - it->second.t_submit;
+  return it->second.status ? it->second.status : s->error;
+}
+
+int wino_server_drain(wino_server* s) {
+  if (!s) return WINO_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv.wait(lk, [&] { return s->stats.completed >= s->stats.submitted; });
+  return s->error;
+}
+
+int wino_server_stats(wino_server* s, wino_server_stats_t* st) {
+  if (!s || !st) return WINO_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *st = s->stats;
+  st->pool_peak = s->eng->pool.peak();
+  return WINO_OK;
+}
+
+int wino_server_destroy(wino_server* s) {
+  if (!s) return WINO_ERR_INVALID;
+  s->stop.store(true);
+  s->cv.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (auto& sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.batch) wino_batch_destroy(sl.batch);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.event) (void)hipEventDestroy(sl.event);
+  }
+  const int err = s->error;
+  delete s;
+  return err;
+}
+
+}  // extern "C"

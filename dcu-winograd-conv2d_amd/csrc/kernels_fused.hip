@@ -29,7 +29,7 @@ namespace wino {
 constexpr int kFusedPB = 32;   // tiles per block
 constexpr int kFusedKB = 64;   // output channels per block
 constexpr int kFusedThreads = 512;
-constexpr int kFusedChanBytes = 768;  // LDS bytes per staged channel slot (largest variant)
+
 
 struct FusedGeom {
   uint32_t N, C, H, W, K;

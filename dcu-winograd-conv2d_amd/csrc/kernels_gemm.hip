@@ -1,5 +1,5 @@
 // kernels_gemm.hip -- the T*T independent GEMMs of the non-fused pipeline
-// as one batched MFMA launch:  M[t] (K x P) = U[t] (K x C) . V[t] (C x P).
+// as batched MFMA work:  M[t] (K x P) = U[t] (K x C) . V[t] (C x P).
 //
 // Replaces the reference's hep_sgemm and its MMAC kernels
 // (include/hep_sgemm.h:495-612, K10-K13; call sites
@@ -14,37 +14,52 @@
 //        the hi/lo split issues hi*hi + hi*lo + lo*hi (+ lo*lo for F(4x3),
 //        whose output transform amplifies the dropped term past 1e-2) with
 //        the lo operand's 2^-4 folded into the MFMA's E8M0 block scale
-//        (123 = 2^-4, 119 = 2^-8, 127 = 1).
-//        The epilogue multiplies by u_scale[t][k] * v_scale[t].
+//        (123 = 2^-4, 127 = 1).  The epilogue multiplies by
+//        u_scale[t][k] * v_scale[t].
+//
+// A tile is a device function so that the single-problem kernels (grid =
+// tiles) and the persistent grouped kernel (kernels_grouped.hip) share it.
+// With a reduction split (split-K), tile `sp` of `split` covers reduction
+// steps [s0, s1) and writes its FP32 partial sums to M32[sp][t][Kp][Pp];
+// the output transform adds the partials in a fixed order.
 #include "wino_internal.h"
 #include "wino_math.h"
 
 namespace wino {
 
-// Block = WK x WP waves; wave tile = (TK*16) x (TP*16); block tile BK x BP.
 template <int WK, int WP, int TK, int TP>
-__global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restrict__ U,
-                                                          const half8* __restrict__ V,
-                                                          half_t* __restrict__ M, uint32_t Kp,
-                                                          uint64_t Pp, uint32_t C8p) {
-  constexpr int NT = WK * WP * 64;
-  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+struct TileShape {
+  static constexpr int NT = WK * WP * 64;
+  static constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+  static constexpr int CS = BP + 8;  // epilogue row stride (halves)
+  static constexpr int lds_f16() {
+    return 2 * 4 * (BK + BP) * 16 > BK * CS * 2 ? 2 * 4 * (BK + BP) * 16 : BK * CS * 2;
+  }
+  static constexpr int lds_f8(int planes) {
+    return 2 * planes * 4 * (BK + BP) * 32 > BK * CS * 2 ? 2 * planes * 4 * (BK + BP) * 32
+                                                         : BK * CS * 2;
+  }
+};
+
+// One BK x BP tile of M[t] over reduction steps [s0, s1) (32 channels each).
+template <int WK, int WP, int TK, int TP>
+__device__ inline void gemm_tile_f16(char* smem, const half8* __restrict__ U,
+                                     const half8* __restrict__ V, half_t* __restrict__ M,
+                                     float* __restrict__ M32, uint32_t Kp, uint64_t Pp,
+                                     uint32_t C8p, uint32_t T2, uint32_t t, uint32_t kb,
+                                     uint64_t pb, int s0, int s1, int sp) {
+  using S = TileShape<WK, WP, TK, TP>;
+  constexpr int NT = S::NT, BK = S::BK, BP = S::BP, CS = S::CS;
   constexpr int ALD = (4 * BK + NT - 1) / NT, BLD = (4 * BP + NT - 1) / NT;
-  constexpr int CS = BP + 8;  // epilogue row stride (halves)
-  constexpr int OPER = 2 * 4 * (BK + BP) * 16;
-  constexpr int EPI = BK * CS * 2;
-  __shared__ __attribute__((aligned(16))) char smem[OPER > EPI ? OPER : EPI];
-  half8* As = (half8*)smem;                 // [2][4][BK]
-  half8* Bs = (half8*)smem + 2 * 4 * BK;    // [2][4][BP]
+  half8* As = (half8*)smem;               // [2][4][BK]
+  half8* Bs = (half8*)smem + 2 * 4 * BK;  // [2][4][BP]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wk = wave % WK, wp = wave / WK;
-  const uint32_t t = blockIdx.z;
-  const uint32_t k0 = blockIdx.y * BK;
-  const uint64_t p0 = (uint64_t)blockIdx.x * BP;
+  const uint32_t k0 = kb * BK;
+  const uint64_t p0 = pb * BP;
   const half8* Ut = U + (uint64_t)t * C8p * Kp + k0;
   const half8* Vt = V + (uint64_t)t * C8p * Pp + p0;
-  const int nsteps = (int)(C8p / 4);
 
   half8 ra[ALD], rb[BLD];
   auto gload = [&](int s) {
@@ -79,12 +94,12 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
     for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   const int g = lane >> 4, r16 = lane & 15;
-  gload(0);
+  gload(s0);
   sstore(0);
   __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < nsteps) gload(s + 1);
+  for (int s = s0; s < s1; ++s) {
+    const int cur = (s - s0) & 1;
+    if (s + 1 < s1) gload(s + 1);
     half8 a[TK], b[TP];
 #pragma unroll
     for (int i = 0; i < TK; ++i) a[i] = As[(cur * 4 + g) * BK + wk * TK * 16 + i * 16 + r16];
@@ -95,10 +110,22 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
 #pragma unroll
       for (int j = 0; j < TP; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
-    if (s + 1 < nsteps) sstore(cur ^ 1);
+    if (s + 1 < s1) sstore(cur ^ 1);
     __syncthreads();
   }
 
+  if (M32) {  // split reduction: FP32 partial sums straight from the accumulators
+    float* Mp = M32 + (((uint64_t)sp * T2 + t) * Kp + k0) * Pp + p0;
+#pragma unroll
+    for (int i = 0; i < TK; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          Mp[(uint64_t)(wk * TK * 16 + i * 16 + 4 * g + e) * Pp + wp * TP * 16 + j * 16 + r16] =
+              acc[i][j][e];
+    return;
+  }
   // epilogue: acc -> fp16 tile in LDS -> 16-byte row stores of M
   half_t* Cs = (half_t*)smem;
 #pragma unroll
@@ -118,35 +145,32 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
     const int row = q / (BP / 8), c8 = q % (BP / 8);
     *(half8*)(Mt + (uint64_t)row * Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
   }
+  __syncthreads();  // LDS is reused by the caller's next tile
 }
 
 // fp8 operands: 32-byte groups [t][c32][rows][32]; PLANES 1 (e4m3) or 2
 // (hi plane followed by lo plane, each T2*C32p*rows*32 bytes).
 template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
-__global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
-    const intx8* __restrict__ U, const intx8* __restrict__ V, half_t* __restrict__ M,
-    const float* __restrict__ us, const float* __restrict__ vs, uint32_t Kp, uint64_t Pp,
-    uint32_t C32p, uint32_t T2) {
-  constexpr int NT = WK * WP * 64;
-  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+__device__ inline void gemm_tile_f8(char* smem, const intx8* __restrict__ U,
+                                    const intx8* __restrict__ V, half_t* __restrict__ M,
+                                    float* __restrict__ M32, const float* __restrict__ us,
+                                    const float* __restrict__ vs, uint32_t Kp, uint64_t Pp,
+                                    uint32_t C32p, uint32_t T2, uint32_t t, uint32_t kb,
+                                    uint64_t pb, int s0, int s1, int sp) {
+  using S = TileShape<WK, WP, TK, TP>;
+  constexpr int NT = S::NT, BK = S::BK, BP = S::BP, CS = S::CS;
   constexpr int ALD = (PLANES * 4 * BK + NT - 1) / NT, BLD = (PLANES * 4 * BP + NT - 1) / NT;
-  constexpr int CS = BP + 8;
-  constexpr int EPI = BK * CS * 2;
-  extern __shared__ __attribute__((aligned(16))) char dsmem[];
-  (void)EPI;
-  intx8* As = (intx8*)dsmem;                          // [2][PLANES][4][BK]
-  intx8* Bs = (intx8*)dsmem + 2 * PLANES * 4 * BK;    // [2][PLANES][4][BP]
+  intx8* As = (intx8*)smem;                        // [2][PLANES][4][BK]
+  intx8* Bs = (intx8*)smem + 2 * PLANES * 4 * BK;  // [2][PLANES][4][BP]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wk = wave % WK, wp = wave / WK;
-  const uint32_t t = blockIdx.z;
-  const uint32_t k0 = blockIdx.y * BK;
-  const uint64_t p0 = (uint64_t)blockIdx.x * BP;
+  const uint32_t k0 = kb * BK;
+  const uint64_t p0 = pb * BP;
   const uint64_t uplane = (uint64_t)T2 * C32p * Kp;  // intx8 entries per plane
   const uint64_t vplane = (uint64_t)T2 * C32p * Pp;
   const intx8* Ut = U + (uint64_t)t * C32p * Kp + k0;
   const intx8* Vt = V + (uint64_t)t * C32p * Pp + p0;
-  const int nsteps = (int)(C32p / 4);
 
   intx8 ra[ALD], rb[BLD];
   auto gload = [&](int s) {
@@ -187,12 +211,12 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
     for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   const int g = lane >> 4, r16 = lane & 15;
-  gload(0);
+  gload(s0);
   sstore(0);
   __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < nsteps) gload(s + 1);
+  for (int s = s0; s < s1; ++s) {
+    const int cur = (s - s0) & 1;
+    if (s + 1 < s1) gload(s + 1);
     intx8 a[PLANES][TK], b[PLANES][TP];
 #pragma unroll
     for (int pl = 0; pl < PLANES; ++pl) {
@@ -219,12 +243,26 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
                 a[PLANES - 1][i], b[PLANES - 1][j], acc[i][j], 0, 0, 0, 123, 0, 123);
         }
       }
-    if (s + 1 < nsteps) sstore(cur ^ 1);
+    if (s + 1 < s1) sstore(cur ^ 1);
     __syncthreads();
   }
 
-  half_t* Cs = (half_t*)dsmem;
   const float sv = vs[t];
+  if (M32) {
+    float* Mp = M32 + (((uint64_t)sp * T2 + t) * Kp + k0) * Pp + p0;
+#pragma unroll
+    for (int i = 0; i < TK; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = wk * TK * 16 + i * 16 + 4 * g + e;
+        const float sc = us[(uint64_t)t * Kp + k0 + row] * sv;
+#pragma unroll
+        for (int j = 0; j < TP; ++j)
+          Mp[(uint64_t)row * Pp + wp * TP * 16 + j * 16 + r16] = acc[i][j][e] * sc;
+      }
+    return;
+  }
+  half_t* Cs = (half_t*)smem;
 #pragma unroll
   for (int i = 0; i < TK; ++i)
 #pragma unroll
@@ -244,17 +282,41 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
     const int row = q / (BP / 8), c8 = q % (BP / 8);
     *(half8*)(Mt + (uint64_t)row * Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
   }
+  __syncthreads();
+}
+
+// ------------------------------------------------------- single problem
+template <int WK, int WP, int TK, int TP>
+__global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restrict__ U,
+                                                          const half8* __restrict__ V,
+                                                          half_t* __restrict__ M, uint32_t Kp,
+                                                          uint64_t Pp, uint32_t C8p,
+                                                          uint32_t T2) {
+  __shared__ __attribute__((aligned(16))) char smem[TileShape<WK, WP, TK, TP>::lds_f16()];
+  gemm_tile_f16<WK, WP, TK, TP>(smem, U, V, M, nullptr, Kp, Pp, C8p, T2, blockIdx.z,
+                                blockIdx.y, blockIdx.x, 0, (int)(C8p / 4), 0);
+}
+
+template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
+__global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
+    const intx8* __restrict__ U, const intx8* __restrict__ V, half_t* __restrict__ M,
+    const float* __restrict__ us, const float* __restrict__ vs, uint32_t Kp, uint64_t Pp,
+    uint32_t C32p, uint32_t T2) {
+  extern __shared__ __attribute__((aligned(16))) char dsmem[];
+  gemm_tile_f8<WK, WP, TK, TP, PLANES, LOLO>(dsmem, U, V, M, nullptr, us, vs, Kp, Pp, C32p, T2,
+                                             blockIdx.z, blockIdx.y, blockIdx.x, 0,
+                                             (int)(C32p / 4), 0);
 }
 
 // ------------------------------------------------------------ tile table
-// Index -> (WK, WP, TK, TP).  The planner picks an entry per problem
-// (planner.cpp); every entry is instantiated below for both precisions.
+// Index -> (WK, WP, TK, TP); every entry is a 256-thread block so that the
+// persistent grouped kernel can run any of them.
 static const int kCfg[][4] = {
-    {2, 2, 4, 4},  // 0: 128 x 128, 256 threads
+    {2, 2, 4, 4},  // 0: 128 x 128
     {2, 2, 2, 4},  // 1:  64 x 128
     {2, 2, 2, 2},  // 2:  64 x  64
     {1, 4, 2, 2},  // 3:  32 x 128
-    {1, 2, 1, 2},  // 4:  16 x  64, 128 threads
+    {1, 4, 1, 1},  // 4:  16 x  64
 };
 int gemm_cfg_count() { return (int)(sizeof(kCfg) / sizeof(kCfg[0])); }
 GemmCfg gemm_cfg(int idx) {
@@ -270,28 +332,27 @@ static void run_f16(const Plan& pl, const void* U, const void* V, half_t* M, hip
   constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
   dim3 grid((uint32_t)(pl.Pp / BP), pl.Kp / BK, pl.t2);
   hipLaunchKernelGGL((k_gemm_f16<WK, WP, TK, TP>), grid, dim3(WK * WP * 64), 0, st,
-                     (const half8*)U, (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8);
+                     (const half8*)U, (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8,
+                     (uint32_t)pl.t2);
 }
 
 template <int WK, int WP, int TK, int TP>
 static void run_f8(const Plan& pl, const void* U, const void* V, half_t* M, const float* us,
                    const float* vs, hipStream_t st) {
-  constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
-  dim3 grid((uint32_t)(pl.Pp / BP), pl.Kp / BK, pl.t2);
-  const size_t oper1 = 2 * 1 * 4 * (BK + BP) * 32, oper2 = 2 * 2 * 4 * (BK + BP) * 32;
-  const size_t epi = (size_t)BK * (BP + 8) * 2;
+  using S = TileShape<WK, WP, TK, TP>;
+  dim3 grid((uint32_t)(pl.Pp / S::BP), pl.Kp / S::BK, pl.t2);
   if (pl.planes == 2) {
     auto kern = pl.m == 4 ? k_gemm_f8<WK, WP, TK, TP, 2, true> : k_gemm_f8<WK, WP, TK, TP, 2, false>;
-    const size_t lds = oper2 > epi ? oper2 : epi;
+    const size_t lds = S::lds_f8(2);
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, grid, dim3(WK * WP * 64), lds, st, (const intx8*)U, (const intx8*)V,
-                       M, us, vs, pl.Kp, pl.Pp, pl.Cp / 32, (uint32_t)pl.t2);
+    hipLaunchKernelGGL(kern, grid, dim3(S::NT), lds, st, (const intx8*)U, (const intx8*)V, M, us,
+                       vs, pl.Kp, pl.Pp, pl.Cp / 32, (uint32_t)pl.t2);
   } else {
     auto kern = k_gemm_f8<WK, WP, TK, TP, 1, false>;
-    const size_t lds = oper1 > epi ? oper1 : epi;
+    const size_t lds = S::lds_f8(1);
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, grid, dim3(WK * WP * 64), lds, st, (const intx8*)U, (const intx8*)V,
-                       M, us, vs, pl.Kp, pl.Pp, pl.Cp / 32, (uint32_t)pl.t2);
+    hipLaunchKernelGGL(kern, grid, dim3(S::NT), lds, st, (const intx8*)U, (const intx8*)V, M, us,
+                       vs, pl.Kp, pl.Pp, pl.Cp / 32, (uint32_t)pl.t2);
   }
 }
 
@@ -303,7 +364,7 @@ hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M
       case 1: run_f16<2, 2, 2, 4>(pl, U, V, M, st); break;
       case 2: run_f16<2, 2, 2, 2>(pl, U, V, M, st); break;
       case 3: run_f16<1, 4, 2, 2>(pl, U, V, M, st); break;
-      default: run_f16<1, 2, 1, 2>(pl, U, V, M, st); break;
+      default: run_f16<1, 4, 1, 1>(pl, U, V, M, st); break;
     }
   } else {
     switch (pl.gemm_cfg) {
@@ -311,10 +372,104 @@ hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M
       case 1: run_f8<2, 2, 2, 4>(pl, U, V, M, u_scale, v_scale, st); break;
       case 2: run_f8<2, 2, 2, 2>(pl, U, V, M, u_scale, v_scale, st); break;
       case 3: run_f8<1, 4, 2, 2>(pl, U, V, M, u_scale, v_scale, st); break;
-      default: run_f8<1, 2, 1, 2>(pl, U, V, M, u_scale, v_scale, st); break;
+      default: run_f8<1, 4, 1, 1>(pl, U, V, M, u_scale, v_scale, st); break;
     }
   }
   return hipGetLastError();
+}
+
+// --------------------------------------------------------- grouped GEMM
+// Persistent kernel over the work items of a batch (kernels_grouped.hip
+// builds the descriptors).  Each item is one tile of one problem; the
+// problem's tile shape and reduction split come from its descriptor.
+__global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __restrict__ hdr,
+                                                          const GProb* __restrict__ probs,
+                                                          const GItem* __restrict__ items) {
+  extern __shared__ __attribute__((aligned(16))) char gsmem[];
+  const uint32_t n = hdr->nitems;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const GItem it = items[i];
+    const GProb& pr = probs[it.prob];
+    const int s0 = (int)(it.sp * pr.nsteps / pr.split);
+    const int s1 = (int)((it.sp + 1) * pr.nsteps / pr.split);
+    float* m32 = pr.split > 1 ? pr.M32 : nullptr;
+    const half8* U = (const half8*)pr.U;
+    const half8* V = (const half8*)pr.V;
+    switch (pr.cfg) {
+      case 0: gemm_tile_f16<2, 2, 4, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+      case 1: gemm_tile_f16<2, 2, 2, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+      case 2: gemm_tile_f16<2, 2, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+      case 3: gemm_tile_f16<1, 4, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+      default: gemm_tile_f16<1, 4, 1, 1>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+    }
+  }
+}
+
+template <int PLANES>
+__global__ __launch_bounds__(256) void k_gemm_grouped_f8(const GroupHeader* __restrict__ hdr,
+                                                         const GProb* __restrict__ probs,
+                                                         const GItem* __restrict__ items) {
+  extern __shared__ __attribute__((aligned(16))) char gsmem[];
+  const uint32_t n = hdr->nitems;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const GItem it = items[i];
+    const GProb& pr = probs[it.prob];
+    const int s0 = (int)(it.sp * pr.nsteps / pr.split);
+    const int s1 = (int)((it.sp + 1) * pr.nsteps / pr.split);
+    float* m32 = pr.split > 1 ? pr.M32 : nullptr;
+    const intx8* U = (const intx8*)pr.U;
+    const intx8* V = (const intx8*)pr.V;
+    const uint32_t c32 = pr.Cp / 32;
+#define WINO_F8_TILE(WK, WP, TK, TP)                                                            \
+  do {                                                                                          \
+    if (PLANES == 2 && pr.m == 4)                                                               \
+      gemm_tile_f8<WK, WP, TK, TP, PLANES, true>(gsmem, U, V, pr.M, m32, pr.us, pr.vs, pr.Kp,   \
+                                                 pr.Pp, c32, pr.t2, it.t, it.kb, it.pb, s0, s1, \
+                                                 it.sp);                                        \
+    else                                                                                        \
+      gemm_tile_f8<WK, WP, TK, TP, PLANES, false>(gsmem, U, V, pr.M, m32, pr.us, pr.vs, pr.Kp,  \
+                                                  pr.Pp, c32, pr.t2, it.t, it.kb, it.pb, s0,    \
+                                                  s1, it.sp);                                   \
+  } while (0)
+    switch (pr.cfg) {
+      case 0: WINO_F8_TILE(2, 2, 4, 4); break;
+      case 1: WINO_F8_TILE(2, 2, 2, 4); break;
+      case 2: WINO_F8_TILE(2, 2, 2, 2); break;
+      case 3: WINO_F8_TILE(1, 4, 2, 2); break;
+      default: WINO_F8_TILE(1, 4, 1, 1); break;
+    }
+#undef WINO_F8_TILE
+  }
+}
+
+size_t grouped_gemm_lds(int precision) {
+  if (precision == WINO_PREC_FP16) return (size_t)TileShape<2, 2, 4, 4>::lds_f16();
+  return (size_t)TileShape<2, 2, 4, 4>::lds_f8(precision == WINO_PREC_FP8X2 ? 2 : 1);
+}
+
+hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
+                               const GItem* items, uint32_t grid, hipStream_t st) {
+  const size_t lds = grouped_gemm_lds(precision);
+  if (precision == WINO_PREC_FP16) {
+    hipLaunchKernelGGL(k_gemm_grouped_f16, dim3(grid), dim3(256), lds, st, hdr, probs, items);
+  } else if (precision == WINO_PREC_FP8) {
+    hipLaunchKernelGGL(k_gemm_grouped_f8<1>, dim3(grid), dim3(256), lds, st, hdr, probs, items);
+  } else {
+    hipLaunchKernelGGL(k_gemm_grouped_f8<2>, dim3(grid), dim3(256), lds, st, hdr, probs, items);
+  }
+  return hipGetLastError();
+}
+
+hipError_t grouped_gemm_prepare(int precision) {
+  const int lds = (int)grouped_gemm_lds(precision);
+  hipError_t e = hipFuncSetAttribute((const void*)k_gemm_grouped_f16,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_gemm_grouped_f8<1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_gemm_grouped_f8<2>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 }  // namespace wino

@@ -1,4 +1,4 @@
-// kernels_transform.hip -- filter, input and output transforms for gfx950.
+// kernels_transform.hip -- filter transform and the fp8 input reduction.
 //
 // All three are HBM-bound streaming kernels: FP32 arithmetic, fp16 (or e4m3)
 // operands written in the GEMM's blocked layout (wino_internal.h).
@@ -143,129 +143,36 @@ __global__ __launch_bounds__(256) void k_amax(const half_t* __restrict__ x, uint
   }
 }
 
-// ------------------------------------------------------------------ input
-
-// A block stages CG channels of a window of WR x WC tiles (plus the 2-pixel
-// halo, zero outside the image) into LDS with coalesced row reads, then each
-// thread transforms one (tile, channel) pair from LDS; consecutive lanes are
-// consecutive channels of one tile, so a wave writes contiguous V rows.
-// PREC: 0 fp16 (CG = 8), 1 fp8, 2 fp8 hi/lo (CG = 32).
-struct InputGeom {
-  uint32_t N, C, H, W;
-  uint32_t TH, TW;     // tile grid
-  uint32_t WR, WC;     // window in tiles
-  uint32_t nwr, nwc;   // windows per image
-  uint32_t CGp;        // channel groups in the padded layout (Cp / CG)
-  uint64_t Pp;
-};
-
-template <int m, int CG, int PREC>
-__global__ __launch_bounds__(256) void k_input(const half_t* __restrict__ X, void* __restrict__ Vout,
-                                               InputGeom g, const float* __restrict__ amax,
-                                               float* __restrict__ vscale) {
-  constexpr int T = m + 2, T2 = T * T;
-  extern __shared__ half_t xs[];
-  uint32_t b = blockIdx.x;
-  const uint32_t wc = b % g.nwc;
-  b /= g.nwc;
-  const uint32_t wr = b % g.nwr;
-  const uint32_t n = b / g.nwr;
-  const uint32_t cg = blockIdx.y;
-  const uint32_t th0 = wr * g.WR, tw0 = wc * g.WC;
-  const uint32_t nth = min(g.WR, g.TH - th0), ntw = min(g.WC, g.TW - tw0);
-  const int LH = (int)nth * m + 2, LW = (int)ntw * m + 2;
-  const int y0 = (int)(th0 * m) - 1, x0 = (int)(tw0 * m) - 1;
-  const int tid = threadIdx.x;
-
-  // stage the window (zero halo / zero channels beyond C)
-  const int per_ch = LH * LW;
-  for (int e = tid; e < CG * per_ch; e += 256) {
-    const int ch = e / per_ch;
-    const int rem = e - ch * per_ch;
-    const int row = rem / LW, col = rem - (rem / LW) * LW;
-    const int y = y0 + row, x = x0 + col;
-    const uint32_t c = cg * CG + ch;
-    half_t v = (half_t)0.0f;
-    if (c < g.C && y >= 0 && y < (int)g.H && x >= 0 && x < (int)g.W)
-      v = X[(((uint64_t)n * g.C + c) * g.H + y) * g.W + x];
-    xs[e] = v;
-  }
-  __syncthreads();
-
-  float inv[T2];
-  if (PREC != 0) {
-    const float a = *amax;
-#pragma unroll
-    for (int t = 0; t < T2; ++t) {
-      const float s = pow2_scale(Tf<m>::gain(t / T) * Tf<m>::gain(t % T) * a);
-      inv[t] = 1.0f / s;
-      if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) vscale[t] = s;
-    }
-  }
-
-  const int cl = tid % CG;
-  const int ntiles = (int)(nth * ntw);
-  const uint64_t tplane = (uint64_t)g.CGp * g.Pp * CG;  // elements per tile position
-  for (int i = tid / CG; i < ntiles; i += 256 / CG) {
-    const int lth = i / (int)ntw, ltw = i - lth * (int)ntw;
-    const half_t* src = xs + cl * per_ch + (lth * m) * LW + ltw * m;
-    float d[T2], v[T2];
-#pragma unroll
-    for (int a = 0; a < T; ++a)
-#pragma unroll
-      for (int c2 = 0; c2 < T; ++c2) d[a * T + c2] = (float)src[a * LW + c2];
-    input2d<m>(d, v);
-    const uint64_t p = ((uint64_t)n * g.TH + th0 + lth) * g.TW + tw0 + ltw;
-    const uint64_t off = ((uint64_t)cg * g.Pp + p) * CG + cl;
-    if (PREC == 0) {
-      half_t* V = (half_t*)Vout;
-#pragma unroll
-      for (int t = 0; t < T2; ++t) V[t * tplane + off] = (half_t)v[t];
-    } else {
-      uint8_t* V = (uint8_t*)Vout;
-#pragma unroll
-      for (int t = 0; t < T2; ++t) {
-        const float x = v[t] * inv[t];
-        const uint32_t hi = f32_to_e4m3(x);
-        V[t * tplane + off] = (uint8_t)hi;
-        if (PREC == 2)
-          V[(uint64_t)T2 * tplane + t * tplane + off] =
-              (uint8_t)f32_to_e4m3((x - e4m3_to_f32(hi)) * 16.0f);
-      }
-    }
-  }
+// Grouped fp8 input reduction: virtual block v of problem p reduces a
+// 1/amax_count slice of its input; one atomic per block.
+__global__ __launch_bounds__(256) void k_amax_zero_grouped(const GroupHeader* __restrict__ hdr,
+                                                           const GProb* __restrict__ probs) {
+  for (uint32_t p = threadIdx.x; p < hdr->nprob; p += blockDim.x) *probs[p].amax = 0.0f;
 }
 
-// ----------------------------------------------------------------- output
-
-// One thread per (k, tile); reads the T*T GEMM results of its tile (a wave
-// reads contiguous runs of M along p) and writes the m x m output patch.
-template <int m>
-__global__ __launch_bounds__(256) void k_output(const half_t* __restrict__ M,
-                                                half_t* __restrict__ Y, uint32_t K, uint32_t TH,
-                                                uint32_t TW, uint64_t P, uint64_t Pp, uint32_t Kp,
-                                                uint32_t OH, uint32_t OW) {
-  constexpr int T = m + 2, T2 = T * T;
-  const uint64_t idx = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  if (idx >= (uint64_t)K * P) return;
-  const uint32_t k = (uint32_t)(idx / P);
-  const uint64_t p = idx - (uint64_t)k * P;
-  const uint32_t per_img = TH * TW;
-  const uint32_t n = (uint32_t)(p / per_img);
-  const uint32_t rem = (uint32_t)(p - (uint64_t)n * per_img);
-  const uint32_t th = rem / TW, tw = rem - (rem / TW) * TW;
-  float x[T2], y[m * m];
-  const uint64_t plane = (uint64_t)Kp * Pp;
-  const half_t* src = M + (uint64_t)k * Pp + p;
+__global__ __launch_bounds__(256) void k_amax_grouped(const GroupHeader* __restrict__ hdr,
+                                                      const GProb* __restrict__ probs) {
+  __shared__ float red[4];
+  const uint32_t total = hdr->amax_blocks, np = hdr->nprob;
+  for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
+    uint32_t p = 0;
+    while (p + 1 < np && probs[p + 1].amax_first <= v) ++p;
+    const GProb& pr = probs[p];
+    const uint32_t local = v - pr.amax_first;
+    const uint64_t n = (uint64_t)pr.ig.N * pr.ig.C * pr.ig.H * pr.ig.W;
+    const uint64_t stride = (uint64_t)pr.amax_count * 256u;
+    float a = 0.0f;
+    for (uint64_t i = (uint64_t)local * 256u + threadIdx.x; i < n; i += stride)
+      a = fmaxf(a, fabsf((float)pr.x[i]));
 #pragma unroll
-  for (int t = 0; t < T2; ++t) x[t] = (float)src[t * plane];
-  output2d<m>(x, y);
-  half_t* dst = Y + (((uint64_t)n * K + k) * OH + th * m) * OW + tw * m;
-#pragma unroll
-  for (int i = 0; i < m; ++i)
-#pragma unroll
-    for (int j = 0; j < m; ++j)
-      if (th * m + i < OH && tw * m + j < OW) dst[i * OW + j] = (half_t)y[i * m + j];
+    for (int o = 32; o >= 1; o >>= 1) a = fmaxf(a, __shfl_xor(a, o));
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      atomicMax((unsigned int*)pr.amax,
+                __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+    __syncthreads();
+  }
 }
 
 // --------------------------------------------------------------- launchers
@@ -313,74 +220,10 @@ hipError_t launch_input_amax(const _Float16* x, uint64_t count, float* amax, hip
   return hipGetLastError();
 }
 
-// Window of WR x WC tiles whose CG-channel LDS image fits the budget.
-static void input_window(const Plan& pl, int CG, uint32_t& WR, uint32_t& WC) {
-  const uint64_t budget = 40u * 1024u;  // bytes of LDS per block
-  WC = pl.tiles_w < 32u ? pl.tiles_w : 32u;
-  for (;;) {
-    const uint64_t lw = (uint64_t)WC * pl.m + 2;
-    const uint64_t rows = budget / ((uint64_t)CG * 2u * lw);
-    if (rows >= (uint64_t)pl.m + 2 || WC == 1) {
-      uint64_t wr = rows > 2 ? (rows - 2) / pl.m : 1;
-      if (wr < 1) wr = 1;
-      // enough tiles for a few passes of the block, no more than the image
-      const uint64_t want = (uint64_t)(4 * 256 / CG + WC - 1) / WC;
-      if (wr > want) wr = want;
-      if (wr > pl.tiles_h) wr = pl.tiles_h;
-      WR = (uint32_t)(wr < 1 ? 1 : wr);
-      return;
-    }
-    WC = (WC + 1) / 2;
-  }
-}
-
-hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N, uint32_t C,
-                                  uint32_t H, uint32_t W, void* V, const float* amax,
-                                  float* v_scale, hipStream_t st) {
-  const int CG = pl.precision == WINO_PREC_FP16 ? 8 : 32;
-  InputGeom g;
-  g.N = N;
-  g.C = C;
-  g.H = H;
-  g.W = W;
-  g.TH = pl.tiles_h;
-  g.TW = pl.tiles_w;
-  input_window(pl, CG, g.WR, g.WC);
-  g.nwr = div_up(g.TH, g.WR);
-  g.nwc = div_up(g.TW, g.WC);
-  g.CGp = pl.Cp / CG;
-  g.Pp = pl.Pp;
-  const size_t lds = (size_t)CG * (g.WR * pl.m + 2) * (g.WC * pl.m + 2) * sizeof(half_t);
-  dim3 grid((uint32_t)((uint64_t)N * g.nwr * g.nwc), div_up(C, CG));
-  if (pl.precision == WINO_PREC_FP16) {
-    if (pl.m == 2)
-      hipLaunchKernelGGL((k_input<2, 8, 0>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-    else
-      hipLaunchKernelGGL((k_input<4, 8, 0>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-  } else if (pl.precision == WINO_PREC_FP8) {
-    if (pl.m == 2)
-      hipLaunchKernelGGL((k_input<2, 32, 1>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-    else
-      hipLaunchKernelGGL((k_input<4, 32, 1>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-  } else {
-    if (pl.m == 2)
-      hipLaunchKernelGGL((k_input<2, 32, 2>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-    else
-      hipLaunchKernelGGL((k_input<4, 32, 2>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N, uint32_t K,
-                                   _Float16* y, hipStream_t st) {
-  (void)N;
-  dim3 grid(div_up((uint64_t)K * pl.P, 256));
-  if (pl.m == 2)
-    hipLaunchKernelGGL(k_output<2>, grid, dim3(256), 0, st, M, y, K, pl.tiles_h, pl.tiles_w, pl.P,
-                       pl.Pp, pl.Kp, pl.oh, pl.ow);
-  else
-    hipLaunchKernelGGL(k_output<4>, grid, dim3(256), 0, st, M, y, K, pl.tiles_h, pl.tiles_w, pl.P,
-                       pl.Pp, pl.Kp, pl.oh, pl.ow);
+hipError_t launch_amax_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_amax_zero_grouped, dim3(1), dim3(256), 0, st, hdr, probs);
+  hipLaunchKernelGGL(k_amax_grouped, dim3(grid), dim3(256), 0, st, hdr, probs);
   return hipGetLastError();
 }
 

@@ -56,6 +56,11 @@ struct Param {
   Plan plan;
 };
 
+// ------------------------------------------------------------ planning
+int reference_select(const problem_t* pr);
+int validate(const problem_t* pr);
+int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision);
+
 // ------------------------------------------------------------ GEMM table
 struct GemmCfg {
   int bk, bp;      // block tile
@@ -63,6 +68,69 @@ struct GemmCfg {
 };
 int gemm_cfg_count();
 GemmCfg gemm_cfg(int idx);
+
+// Input-transform window geometry (kernels_transform.hip).
+struct InputGeom {
+  uint32_t N, C, H, W;
+  uint32_t TH, TW;     // tile grid
+  uint32_t WR, WC;     // window in tiles
+  uint32_t nwr, nwc;   // windows per image
+  uint32_t CGp;        // channel groups in the padded layout (Cp / CG)
+  uint64_t Pp;
+};
+// Window of WR x WC tiles whose CG-channel LDS image fits `budget` bytes.
+void input_window(const Plan& pl, int CG, uint64_t budget, uint32_t& WR, uint32_t& WC);
+InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_t W,
+                     uint64_t budget);
+
+// ------------------------------------------------- grouped execution
+// A batch of independent problems executed by three fixed-grid launches
+// (input transform, persistent GEMM, output transform) that read their work
+// from these device-resident descriptors; the host rewrites the descriptors
+// between batches, so one captured graph serves every batch.
+struct GroupHeader {
+  uint32_t nprob;
+  uint32_t in_blocks;    // total virtual blocks of the input transform
+  uint32_t out_blocks;   // total virtual blocks of the output transform
+  uint32_t nitems;       // GEMM work items
+  uint32_t amax_blocks;  // total virtual blocks of the fp8 input reduction
+  uint32_t pad[3];
+};
+struct GProb {
+  const _Float16* x;
+  _Float16* y;
+  void* U;
+  void* V;
+  _Float16* M;
+  float* M32;            // split partial sums [split][t2][Kp][Pp] (split > 1)
+  float* us;             // fp8 scales
+  float* vs;
+  float* amax;
+  InputGeom ig;
+  uint64_t P, Pp;
+  uint32_t K, Kp, Cp, m, t2, OH, OW;
+  uint32_t cfg, split, nsteps;
+  uint32_t in_gx;        // input-transform blocks along x (then ceil(C/CG) along y)
+  uint32_t in_first, in_count;
+  uint32_t out_first, out_count;
+  uint32_t amax_first, amax_count;
+};
+struct GItem {
+  uint32_t prob, t, kb, sp;
+  uint64_t pb;
+};
+constexpr uint32_t kGroupInputLds = 48 * 1024;  // dynamic LDS of the grouped input kernel
+
+hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
+                               const GItem* items, uint32_t grid, hipStream_t st);
+hipError_t grouped_gemm_prepare(int precision);
+size_t grouped_gemm_lds(int precision);
+hipError_t launch_input_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
+                                uint32_t grid, hipStream_t st);
+hipError_t launch_output_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
+                                 hipStream_t st);
+hipError_t launch_amax_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
+                               hipStream_t st);
 
 // ------------------------------------------------------------ launchers
 // All return hipSuccess or the launch error; they only enqueue.

@@ -55,6 +55,13 @@ class Info(C.Structure):
                 ("u_scale", C.c_void_p), ("v_scale", C.c_void_p)]
 
 
+class ServerStats(C.Structure):
+    """wino_server_stats_t, include/winoconv.h."""
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("batches", C.c_uint64), ("max_inflight_batches", C.c_uint32),
+                ("max_batch_problems", C.c_uint32), ("pool_peak", C.c_size_t)]
+
+
 # name -> (restype, argtypes); every symbol include/*.h declares
 _P = C.c_void_p
 _SIGS = {
@@ -83,6 +90,32 @@ _SIGS = {
     "wino_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
     "wino_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
     "wino_device_synchronize": (C.c_int, []),
+    "wino_engine_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_size_t]),
+    "wino_engine_destroy": (C.c_int, [_P]),
+    "wino_engine_load_tuning": (C.c_int, [_P, C.c_char_p]),
+    "wino_engine_register_filter": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_int,
+                                               C.POINTER(C.c_int)]),
+    "wino_engine_refresh_filter": (C.c_int, [_P, C.c_int]),
+    "wino_engine_footprint": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.POINTER(C.c_size_t)]),
+    "wino_engine_pool_usage": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "wino_batch_create": (C.c_int, [_P, C.c_int, C.POINTER(C.c_void_p)]),
+    "wino_batch_destroy": (C.c_int, [_P]),
+    "wino_batch_add": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_int, C.POINTER(C.c_int)]),
+    "wino_batch_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "wino_batch_launch": (C.c_int, [_P, _P]),
+    "wino_batch_capture": (C.c_int, [_P, _P]),
+    "wino_batch_reset": (C.c_int, [_P]),
+    "wino_batch_info": (C.c_int, [_P, C.c_int, C.POINTER(Info)]),
+    "wino_batch_size": (C.c_int, [_P]),
+    "wino_server_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "wino_server_destroy": (C.c_int, [_P]),
+    "wino_server_submit": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.POINTER(C.c_uint64)]),
+    "wino_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
+    "wino_server_drain": (C.c_int, [_P]),
+    "wino_server_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
 }
 
 _lib = None
@@ -206,3 +239,119 @@ class Conv2d:
             self.destroy()
         except Exception:
             pass
+
+
+class Engine:
+    """Resident-filter engine: a device workspace pool plus registered filters
+    whose Winograd transforms stay on the device (include/winoconv.h)."""
+
+    def __init__(self, precision: int = PREC_FP16, workspace_bytes: int = 1 << 30,
+                 tuning: str | None = None):
+        h = C.c_void_p()
+        check(lib().wino_engine_create(C.byref(h), precision, workspace_bytes), "engine_create")
+        self.handle = h
+        self.precision = precision
+        if tuning:
+            check(lib().wino_engine_load_tuning(h, tuning.encode()), "load_tuning")
+
+    def register_filter(self, w: int, k: int, c: int, algo: int = ALGO_AUTO) -> int:
+        fid = C.c_int(-1)
+        check(lib().wino_engine_register_filter(self.handle, w, k, c, algo, C.byref(fid)),
+              "register_filter")
+        return fid.value
+
+    def refresh_filter(self, fid: int) -> None:
+        check(lib().wino_engine_refresh_filter(self.handle, fid), "refresh_filter")
+
+    def footprint(self, fid: int, n: int, h: int, w: int) -> int:
+        b = C.c_size_t(0)
+        check(lib().wino_engine_footprint(self.handle, fid, n, h, w, C.byref(b)), "footprint")
+        return b.value
+
+    def pool_usage(self) -> tuple[int, int]:
+        u, cap = C.c_size_t(0), C.c_size_t(0)
+        check(lib().wino_engine_pool_usage(self.handle, C.byref(u), C.byref(cap)), "pool_usage")
+        return u.value, cap.value
+
+    def destroy(self) -> None:
+        if getattr(self, "handle", None):
+            lib().wino_engine_destroy(self.handle)
+            self.handle = None
+
+
+class Batch:
+    """A grouped launch of independent problems (include/winoconv.h)."""
+
+    def __init__(self, engine: Engine, max_problems: int = 64):
+        h = C.c_void_p()
+        check(lib().wino_batch_create(engine.handle, max_problems, C.byref(h)), "batch_create")
+        self.handle = h
+        self.engine = engine
+
+    def add(self, fid: int, x: int, y: int, n: int, h: int, w: int, algo: int = ALGO_AUTO) -> int:
+        idx = C.c_int(-1)
+        check(lib().wino_batch_add(self.handle, fid, x, y, n, h, w, algo, C.byref(idx)), "batch_add")
+        return idx.value
+
+    def set_tile(self, index: int, cfg: int, split: int = 1) -> None:
+        check(lib().wino_batch_set_tile(self.handle, index, cfg, split), "batch_set_tile")
+
+    def launch(self, stream: int | None = None) -> None:
+        check(lib().wino_batch_launch(self.handle, stream or None), "batch_launch")
+
+    def capture(self, stream: int) -> None:
+        check(lib().wino_batch_capture(self.handle, stream), "batch_capture")
+
+    def reset(self) -> None:
+        check(lib().wino_batch_reset(self.handle), "batch_reset")
+
+    def info(self, index: int) -> Info:
+        inf = Info()
+        check(lib().wino_batch_info(self.handle, index, C.byref(inf)), "batch_info")
+        return inf
+
+    def __len__(self) -> int:
+        return lib().wino_batch_size(self.handle)
+
+    def destroy(self) -> None:
+        if getattr(self, "handle", None):
+            lib().wino_batch_destroy(self.handle)
+            self.handle = None
+
+
+class Server:
+    """Continuous-batching server over an engine (include/winoconv.h)."""
+
+    def __init__(self, engine: Engine, slots: int = 2, max_batch: int = 16, use_graph: bool = False):
+        h = C.c_void_p()
+        check(lib().wino_server_create(engine.handle, slots, max_batch, int(use_graph), C.byref(h)),
+              "server_create")
+        self.handle = h
+
+    def submit(self, fid: int, x: int, y: int, n: int, h: int, w: int) -> int:
+        rid = C.c_uint64(0)
+        check(lib().wino_server_submit(self.handle, fid, x, y, n, h, w, C.byref(rid)), "submit")
+        return rid.value
+
+    def poll(self, rid: int) -> float | None:
+        """Latency in microseconds when done, None while pending."""
+        lat = C.c_double(0.0)
+        st = lib().wino_server_poll(self.handle, rid, C.byref(lat))
+        if st == ERR_BUSY:
+            return None
+        check(st, "poll")
+        return lat.value
+
+    def drain(self) -> None:
+        check(lib().wino_server_drain(self.handle), "drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        check(lib().wino_server_stats(self.handle, C.byref(s)), "stats")
+        return s
+
+    def destroy(self) -> None:
+        if getattr(self, "handle", None):
+            st = lib().wino_server_destroy(self.handle)
+            self.handle = None
+            check(st, "server_destroy")

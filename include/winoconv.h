@@ -116,6 +116,82 @@ int wino_stage_input(const void* param);
 int wino_stage_gemm(const void* param);
 int wino_stage_output(const void* param);
 
+/* ------------------------------------------------ engine: resident filters
+ * An engine owns a device workspace pool (the HBM budget for per-problem
+ * transformed inputs and GEMM results) and a set of registered filters whose
+ * transforms stay resident on the device.  Filters are transformed once, at
+ * registration (the reference re-transformed on every call,
+ * src/winograd_4x3_non_fused.cpp:584).  No counterpart in the reference. */
+typedef struct wino_engine wino_engine;
+int wino_engine_create(wino_engine** engine, int precision, size_t workspace_bytes);
+int wino_engine_destroy(wino_engine* engine);
+/* Tuning table written by the autotuner: one line per shape class,
+ * "m K C P_min P_max cfg split" (whitespace separated, '#' comments). */
+int wino_engine_load_tuning(wino_engine* engine, const char* path);
+/* algo: WINO_ALGO_F23, WINO_ALGO_F43 or WINO_ALGO_AUTO (keep both, choose
+ * per problem with the reference rule; its fused choice runs as F(2x3)). */
+int wino_engine_register_filter(wino_engine* engine, const void* weight, uint32_t k, uint32_t c,
+                                int algo, int* filter_id);
+/* Re-transforms a registered filter after its weights changed. */
+int wino_engine_refresh_filter(wino_engine* engine, int filter_id);
+/* Workspace bytes one problem needs (the admission footprint). */
+int wino_engine_footprint(wino_engine* engine, int filter_id, uint32_t n, uint32_t h, uint32_t w,
+                          size_t* bytes);
+int wino_engine_pool_usage(wino_engine* engine, size_t* used, size_t* capacity);
+
+/* ------------------------------------------- batch: one grouped launch
+ * A batch holds up to max_problems independent 3x3 stride-1 pad-1 convs of
+ * registered filters, each with its own N, H, W; F(2x3) or F(4x3) per
+ * problem; GEMM tile shape and reduction split per problem from the tuning
+ * table.  wino_batch_launch enqueues three fixed-grid kernels (grouped input
+ * transform, one persistent grouped GEMM, grouped output transform) that read
+ * the batch's descriptors from device memory, so a batch captured once in a
+ * hipGraph (wino_batch_capture) replays for any later contents.  A problem's
+ * result does not depend on the other problems of its batch. */
+typedef struct wino_batch wino_batch;
+int wino_batch_create(wino_engine* engine, int max_problems, wino_batch** batch);
+int wino_batch_destroy(wino_batch* batch);
+/* Adds a problem; allocates its workspace from the engine pool
+ * (WINO_ERR_CAPACITY when the pool cannot hold it now). algo: WINO_ALGO_AUTO
+ * or the filter's algorithm. */
+int wino_batch_add(wino_batch* batch, int filter_id, const void* in, void* out, uint32_t n,
+                   uint32_t h, uint32_t w, int algo, int* index);
+/* Overrides the tile configuration of problem `index` (autotuning). */
+int wino_batch_set_tile(wino_batch* batch, int index, int cfg, int split);
+int wino_batch_launch(wino_batch* batch, void* stream);
+/* Captures the batch's kernel sequence into a hipGraph on `stream`; later
+ * wino_batch_launch calls replay it. */
+int wino_batch_capture(wino_batch* batch, void* stream);
+/* Releases the problems' workspace (after the launch completed) for reuse. */
+int wino_batch_reset(wino_batch* batch);
+int wino_batch_info(wino_batch* batch, int index, wino_info_t* info);
+int wino_batch_size(wino_batch* batch);
+
+/* ------------------------------------------------ continuous-batching server
+ * A host thread admits submitted requests (one conv of a registered filter)
+ * as workspace and batch slots free up, launches each admitted set as one
+ * grouped batch on one of `slots` streams (batches overlap on the device),
+ * and retires them when their event completes.  Admission is capped by the
+ * engine pool (HBM footprint of V/M); a request larger than the whole pool
+ * is refused at submit.  No counterpart in the reference. */
+typedef struct wino_server wino_server;
+int wino_server_create(wino_engine* engine, int slots, int max_batch, int use_graph,
+                       wino_server** server);
+int wino_server_destroy(wino_server* server);
+int wino_server_submit(wino_server* server, int filter_id, const void* in, void* out, uint32_t n,
+                       uint32_t h, uint32_t w, uint64_t* request_id);
+/* WINO_OK (and latency in microseconds, submit to completion) when done,
+ * WINO_ERR_BUSY while pending. */
+int wino_server_poll(wino_server* server, uint64_t request_id, double* latency_us);
+int wino_server_drain(wino_server* server);
+typedef struct {
+  uint64_t submitted, completed, rejected, batches;
+  uint32_t max_inflight_batches;  /* most batches on the device at once */
+  uint32_t max_batch_problems;    /* largest batch launched               */
+  size_t pool_peak;               /* most pool bytes in use at once       */
+} wino_server_stats_t;
+int wino_server_stats(wino_server* server, wino_server_stats_t* stats);
+
 /* ------------------------------------------------- device helpers
  * Thin wrappers so that hosts without a HIP binding can move data. */
 int wino_device_malloc(void** ptr, size_t bytes);

@@ -23,6 +23,13 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def wc():
+    # PyTorch-ROCm ships its own HIP runtime under the same soname: importing
+    # torch first makes libwinoconv bind to that one runtime instead of
+    # loading a second runtime that would compete for the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     import winoconv
     winoconv.lib()
     return winoconv

@@ -1,0 +1,327 @@
+"""GPU tests of the grouped (batched) launch and the continuous-batching
+server, through the C ABI.
+
+Invariants checked (each would catch a distinct mistake):
+  * a batched problem equals the single-problem operator bit for bit when its
+    reduction is not split (same transforms, same per-element MFMA
+    accumulation order; only the padding and the scheduling differ);
+  * every GEMM tile shape gives bitwise-identical results;
+  * a problem's result does not depend on the rest of its batch (alone vs in
+    a mix, bitwise), including with a split reduction;
+  * a hipGraph captured once replays correctly for later batch contents;
+  * against the FP64 direct convolution, the same rel-L2 bounds as the
+    single-problem tests (fp16: F(2x3) 2e-3, F(4x3) 1e-2; fp8x2: 5e-3 / 1.5e-2).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rng_inputs
+
+pytestmark = pytest.mark.gpu
+
+F43, F23, AUTO = 0, 1, -1
+FP16, FP8, FP8X2 = 0, 1, 2
+TOL = {(2, FP16): 2e-3, (4, FP16): 1e-2, (2, FP8X2): 5e-3, (4, FP8X2): 1.5e-2}
+
+# filters (k, c, algo); problems (filter index, n, h, w)
+FILTERS = [(64, 32, AUTO), (40, 17, F43), (27, 3, F23), (96, 130, AUTO)]
+PROBLEMS = [(0, 1, 56, 56), (0, 2, 7, 9), (1, 3, 14, 14), (1, 1, 5, 5), (2, 2, 13, 11),
+            (2, 1, 1, 1), (3, 2, 10, 10), (3, 4, 28, 28), (0, 8, 16, 16)]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+class Setup:
+    """Host inputs, device buffers and a registered engine for FILTERS/PROBLEMS."""
+
+    def __init__(self, wc, gpu, precision, pool=1 << 30, seed=11):
+        self.wc, self.gpu = wc, gpu
+        self.eng = wc.Engine(precision, pool)
+        self.w_host, self.w_dev, self.fids = [], [], []
+        for i, (k, c, algo) in enumerate(FILTERS):
+            _, wt = rng_inputs(seed + i, 1, c, 1, 1, k, dist="clip4")
+            dw = gpu.to_dev(wt)
+            self.w_host.append(wt)
+            self.w_dev.append(dw)
+            self.fids.append(self.eng.register_filter(dw.data_ptr(), k, c, algo))
+        self.x_host, self.x_dev, self.y_dev = [], [], []
+        for j, (fi, n, h, w) in enumerate(PROBLEMS):
+            k, c, _ = FILTERS[fi]
+            x, _ = rng_inputs(seed + 100 + j, n, c, h, w, 1, dist="clip4")
+            self.x_host.append(x)
+            self.x_dev.append(gpu.to_dev(x))
+            self.y_dev.append(gpu.empty_f16((n, k, h, w)))
+        # torch fills on its stream; batches may run on other, non-blocking streams
+        import torch
+        torch.cuda.synchronize()
+
+    def add(self, batch, j, y=None, algo=AUTO):
+        fi, n, h, w = PROBLEMS[j]
+        y = self.y_dev[j] if y is None else y
+        return batch.add(self.fids[fi], self.x_dev[j].data_ptr(), y.data_ptr(), n, h, w, algo)
+
+    def fresh_out(self, j):
+        import torch
+        fi, n, h, w = PROBLEMS[j]
+        y = self.gpu.empty_f16((n, FILTERS[fi][0], h, w))
+        torch.cuda.synchronize()
+        return y
+
+    def close(self):
+        self.eng.destroy()
+
+
+def _m_of(batch, idx):
+    return batch.info(idx).m
+
+
+def test_batch_matches_single_problem_and_fp64(wc, gpu, oracle):
+    S = Setup(wc, gpu, FP16)
+    b = wc.Batch(S.eng, 32)
+    try:
+        for j in range(len(PROBLEMS)):
+            idx = S.add(b, j)
+            b.set_tile(idx, 2, 1)  # no reduction split: bitwise parity with the single path
+        say(f"grouped fp16: {len(b)} problems")
+        b.launch()
+        for j, (fi, n, h, w) in enumerate(PROBLEMS):
+            k, c, _ = FILTERS[fi]
+            m = _m_of(b, j)
+            y = gpu.to_host(S.y_dev[j])
+            e = oracle.errors(y.ravel().astype(np.float64),
+                              oracle.conv2d_f64(S.x_host[j], S.w_host[fi]).ravel())
+            say(f"  problem {j} {PROBLEMS[j]} m={m} relL2={e[1]:.3e}")
+            assert np.isfinite(y).all()
+            assert e[1] <= TOL[(m, FP16)], (j, e)
+            # single-problem operator, same algorithm
+            ys, _ = gpu.run_conv(wc, S.x_host[j], S.w_host[fi], algo=F43 if m == 4 else F23)
+            assert np.array_equal(ys.view(np.uint16), y.view(np.uint16)), j
+    finally:
+        b.destroy()
+        S.close()
+
+
+def test_tile_shapes_bitwise_equal(wc, gpu):
+    S = Setup(wc, gpu, FP16)
+    b = wc.Batch(S.eng, 8)
+    try:
+        j = 7  # (96 x 130) filter, 4 x 28 x 28
+        outs = []
+        for cfg in range(5):
+            b.reset()
+            y = S.fresh_out(j)
+            idx = S.add(b, j, y)
+            b.set_tile(idx, cfg, 1)
+            say(f"tile cfg={cfg}")
+            b.launch()
+            outs.append(gpu.to_host(y).view(np.uint16))
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+    finally:
+        b.destroy()
+        S.close()
+
+
+@pytest.mark.parametrize("precision", [FP16, FP8X2])
+def test_alone_vs_mix_bitwise(wc, gpu, oracle, precision):
+    """Default plans (tuned tile, split reduction for small problems)."""
+    S = Setup(wc, gpu, precision)
+    mix = wc.Batch(S.eng, 32)
+    solo = wc.Batch(S.eng, 1)
+    try:
+        for j in range(len(PROBLEMS)):
+            S.add(mix, j)
+        splits = [mix.info(j).scale_bytes for j in range(len(PROBLEMS))]
+        say(f"mix precision={precision} splits={splits}")
+        mix.launch()
+        ymix = [gpu.to_host(y).view(np.uint16).copy() for y in S.y_dev]
+        for j, (fi, n, h, w) in enumerate(PROBLEMS):
+            solo.reset()
+            y = S.fresh_out(j)
+            S.add(solo, j, y)
+            solo.launch()
+            yo = gpu.to_host(y)
+            assert np.array_equal(yo.view(np.uint16), ymix[j]), j
+            m = solo.info(0).m
+            e = oracle.errors(yo.ravel().astype(np.float64),
+                              oracle.conv2d_f64(S.x_host[j], S.w_host[fi]).ravel())
+            say(f"  problem {j} m={m} split={splits[j]} relL2={e[1]:.3e}")
+            assert e[1] <= TOL[(m, precision)], (j, e)
+    finally:
+        solo.destroy()
+        mix.destroy()
+        S.close()
+
+
+def test_fp8x2_batch_matches_single_problem(wc, gpu):
+    S = Setup(wc, gpu, FP8X2)
+    b = wc.Batch(S.eng, 32)
+    try:
+        for j in range(len(PROBLEMS)):
+            S.add(b, j)
+            b.set_tile(j, 2, 1)
+        b.launch()
+        for j, (fi, n, h, w) in enumerate(PROBLEMS):
+            m = b.info(j).m
+            y = gpu.to_host(S.y_dev[j])
+            ys, _ = gpu.run_conv(wc, S.x_host[j], S.w_host[fi], algo=F43 if m == 4 else F23,
+                                 precision=FP8X2)
+            assert np.array_equal(ys.view(np.uint16), y.view(np.uint16)), j
+    finally:
+        b.destroy()
+        S.close()
+
+
+def test_split_reduction_deterministic(wc, gpu, oracle):
+    S = Setup(wc, gpu, FP16)
+    b = wc.Batch(S.eng, 4)
+    try:
+        j = 6  # C = 130 -> Cp = 160, 5 reduction steps
+        fi = PROBLEMS[j][0]
+        ref = oracle.conv2d_f64(S.x_host[j], S.w_host[fi]).ravel()
+        for split in (2, 3, 5):
+            outs = []
+            for _ in range(2):
+                b.reset()
+                y = S.fresh_out(j)
+                idx = S.add(b, j, y)
+                b.set_tile(idx, 3, split)
+                b.launch()
+                outs.append(gpu.to_host(y))
+            assert np.array_equal(outs[0].view(np.uint16), outs[1].view(np.uint16))
+            e = oracle.errors(outs[0].ravel().astype(np.float64), ref)
+            say(f"split={split} relL2={e[1]:.3e}")
+            assert e[1] <= TOL[(b.info(0).m, FP16)]
+    finally:
+        b.destroy()
+        S.close()
+
+
+def test_graph_replay_new_contents(wc, gpu):
+    """One capture, several replays with different batch contents and eager
+    work between them; every replay equals the eager launch."""
+    import torch
+    S = Setup(wc, gpu, FP16)
+    eager = wc.Batch(S.eng, 16)
+    graph = wc.Batch(S.eng, 16)
+    stream = torch.cuda.Stream()
+    try:
+        sets = [[0, 2, 4], [1, 3, 5, 6, 7], [8], [0, 1, 2, 3, 4, 5, 6, 7, 8]]
+        captured = False
+        for r, js in enumerate(sets):
+            ye = [S.fresh_out(j) for j in js]
+            yg = [S.fresh_out(j) for j in js]
+            eager.reset()
+            graph.reset()
+            for j, a, g in zip(js, ye, yg):
+                S.add(eager, j, a)
+                S.add(graph, j, g)
+            eager.launch()
+            if not captured:
+                graph.capture(stream.cuda_stream)
+                captured = True
+            say(f"graph replay {r}: problems {js}")
+            graph.launch(stream.cuda_stream)
+            stream.synchronize()
+            # eager work between replays
+            junk = torch.randn(1 << 20, device="cuda:0").sum()
+            torch.cuda.synchronize()
+            del junk
+            for a, g in zip(ye, yg):
+                assert np.array_equal(gpu.to_host(a).view(np.uint16), gpu.to_host(g).view(np.uint16))
+    finally:
+        stream.synchronize()
+        graph.destroy()
+        eager.destroy()
+        S.close()
+
+
+def test_pool_capacity_and_release(wc, gpu):
+    S = Setup(wc, gpu, FP16, pool=8 << 20)
+    b = wc.Batch(S.eng, 16)
+    try:
+        fi, n, h, w = PROBLEMS[7]
+        need = S.eng.footprint(S.fids[fi], n, h, w)
+        used0, cap = S.eng.pool_usage()
+        assert used0 == 0 and cap == 8 << 20
+        added = 0
+        while True:
+            try:
+                S.add(b, 3)  # small problem
+                added += 1
+            except wc.WinoError as e:
+                assert e.status == wc.ERR_CAPACITY
+                break
+            if added > 10000:
+                raise AssertionError("pool never filled")
+        used, _ = S.eng.pool_usage()
+        assert 0 < used <= cap
+        b.reset()
+        assert S.eng.pool_usage()[0] == 0
+        big = S.eng.footprint(S.fids[fi], 64, 56, 56)
+        assert big > cap
+        with pytest.raises(wc.WinoError):
+            b.add(S.fids[fi], S.x_dev[7].data_ptr(), S.y_dev[7].data_ptr(), 64, 56, 56)
+        say(f"pool: {added} small problems fit 8 MiB; big footprint {big}; need {need}")
+    finally:
+        b.destroy()
+        S.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_server_poisson_matches_solo(wc, gpu, oracle, use_graph):
+    """Poisson arrivals into the serving loop: every result equals the same
+    problem launched alone (bitwise), a sample equals FP64 within tolerance,
+    and admission never exceeds the pool."""
+    S = Setup(wc, gpu, FP16, pool=64 << 20)
+    solo = wc.Batch(S.eng, 1)
+    ref = []
+    try:
+        for j in range(len(PROBLEMS)):
+            solo.reset()
+            y = S.fresh_out(j)
+            S.add(solo, j, y)
+            solo.launch()
+            ref.append(gpu.to_host(y).view(np.uint16).copy())
+        solo.reset()
+        srv = wc.Server(S.eng, slots=2, max_batch=6, use_graph=use_graph)
+        rng = np.random.default_rng(5)
+        reqs = []
+        try:
+            qps = 4000.0
+            t = time.perf_counter()
+            for r in range(60):
+                j = int(rng.integers(len(PROBLEMS)))
+                y = S.fresh_out(j)
+                fi, n, h, w = PROBLEMS[j]
+                rid = srv.submit(S.fids[fi], S.x_dev[j].data_ptr(), y.data_ptr(), n, h, w)
+                reqs.append((rid, j, y))
+                t += rng.exponential(1.0 / qps)
+                while time.perf_counter() < t:
+                    pass
+            srv.drain()
+            st = srv.stats()
+            lat = [srv.poll(rid) for rid, _, _ in reqs]
+            say(f"server graph={use_graph}: batches={st.batches} max_batch={st.max_batch_problems} "
+                f"inflight={st.max_inflight_batches} pool_peak={st.pool_peak} "
+                f"p50={np.percentile(lat, 50):.0f}us p99={np.percentile(lat, 99):.0f}us")
+            assert st.submitted == 60 and st.completed == 60 and st.rejected == 0
+            assert st.pool_peak <= 64 << 20
+        finally:
+            srv.destroy()
+        for i, (rid, j, y) in enumerate(reqs):
+            yh = gpu.to_host(y)
+            assert np.array_equal(yh.view(np.uint16), ref[j]), (i, j)
+        for i in (0, 17, 41):
+            _, j, y = reqs[i]
+            fi = PROBLEMS[j][0]
+            e = oracle.errors(gpu.to_host(y).ravel().astype(np.float64),
+                              oracle.conv2d_f64(S.x_host[j], S.w_host[fi]).ravel())
+            assert e[1] <= 1e-2, (i, j, e)
+    finally:
+        solo.destroy()
+        S.close()
