@@ -500,4 +500,11 @@ int wino_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 }
 int wino_device_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
 
+int wino_quantize_e4m3(const void* x, void* q, uint64_t n) {
+  if ((!x || !q) && n) return WINO_ERR_INVALID;
+  int st = hip_status(wino::launch_quantize_e4m3((const float*)x, (uint8_t*)q, n, nullptr));
+  if (st) return st;
+  return hip_status(hipDeviceSynchronize());
+}
+
 }  // extern "C"

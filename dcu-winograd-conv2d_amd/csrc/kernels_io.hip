@@ -105,31 +105,30 @@ __device__ inline void input_block(const half_t* __restrict__ X, void* __restric
 #pragma unroll
       for (int t = 0; t < T2; ++t) *(half2_t*)(V + t * tplane + off) = out[t];
     } else {
-      uint32_t hi4[T2], lo4[T2];
-#pragma unroll
-      for (int t = 0; t < T2; ++t) hi4[t] = lo4[t] = 0u;
+      float vq[4][T2];
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
         const half_t* src = xs + (cq * 4 + cc) * per_ch + (lth * m) * LW + ltw * m;
-        float d[T2], v[T2];
+        float d[T2];
 #pragma unroll
         for (int a = 0; a < T; ++a)
 #pragma unroll
           for (int c2 = 0; c2 < T; ++c2) d[a * T + c2] = (float)src[a * LW + c2];
-        input2d<m>(d, v);
-#pragma unroll
-        for (int t = 0; t < T2; ++t) {
-          const float xq = v[t] * inv[t];
-          const uint32_t h = f32_to_e4m3(xq);
-          hi4[t] |= h << (8 * cc);
-          if (PREC == 2) lo4[t] |= f32_to_e4m3((xq - e4m3_to_f32(h)) * 16.0f) << (8 * cc);
-        }
+        input2d<m>(d, vq[cc]);
       }
       uint8_t* V = (uint8_t*)Vout;
 #pragma unroll
       for (int t = 0; t < T2; ++t) {
-        *(uint32_t*)(V + t * tplane + off) = hi4[t];
-        if (PREC == 2) *(uint32_t*)(V + (uint64_t)T2 * tplane + t * tplane + off) = lo4[t];
+        const float x4[4] = {vq[0][t] * inv[t], vq[1][t] * inv[t], vq[2][t] * inv[t],
+                             vq[3][t] * inv[t]};
+        if (PREC == 2) {
+          uint32_t hi, lo;
+          e4m3x4_split(x4, hi, lo);
+          *(uint32_t*)(V + t * tplane + off) = hi;
+          *(uint32_t*)(V + (uint64_t)T2 * tplane + t * tplane + off) = lo;
+        } else {
+          *(uint32_t*)(V + t * tplane + off) = e4m3x4(x4[0], x4[1], x4[2], x4[3]);
+        }
       }
     }
   }

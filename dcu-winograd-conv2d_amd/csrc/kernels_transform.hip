@@ -82,28 +82,53 @@ __global__ __launch_bounds__(64) void k_filter_f8(const half_t* __restrict__ W,
     inv[t] = 1.0f / s;  // exact: s is a power of two
     if (lane == 0) us[(uint64_t)t * Kp + k] = s;
   }
+  // pass 2: four consecutive channels per thread, one dword per tile
+  // position and plane
   const uint64_t plane = (uint64_t)T2 * C32p * Kp * 32u;
-  for (uint32_t c = lane; c < C32p * 32u; c += 64u) {
-    float u[T2];
-    if (k < K && c < C) {
-      const half_t* src = W + ((uint64_t)k * C + c) * 9u;
-      float g[9];
+  for (uint32_t c0 = 4u * lane; c0 < C32p * 32u; c0 += 256u) {
+    float u[4][T2];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) g[i] = (float)src[i];
-      filter2d<m>(g, u);
-    } else {
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t c = c0 + j;
+      if (k < K && c < C) {
+        const half_t* src = W + ((uint64_t)k * C + c) * 9u;
+        float g[9];
 #pragma unroll
-      for (int t = 0; t < T2; ++t) u[t] = 0.0f;
+        for (int i = 0; i < 9; ++i) g[i] = (float)src[i];
+        filter2d<m>(g, u[j]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < T2; ++t) u[j][t] = 0.0f;
+      }
     }
-    const uint64_t base = ((uint64_t)(c >> 5) * Kp + k) * 32u + (c & 31u);
+    const uint64_t base = ((uint64_t)(c0 >> 5) * Kp + k) * 32u + (c0 & 31u);
 #pragma unroll
     for (int t = 0; t < T2; ++t) {
-      const float x = u[t] * inv[t];
-      const uint32_t hi = f32_to_e4m3(x);
-      const uint64_t off = (uint64_t)t * C32p * Kp * 32u + base;
-      U[off] = (uint8_t)hi;
-      if (PLANES == 2) U[plane + off] = (uint8_t)f32_to_e4m3((x - e4m3_to_f32(hi)) * 16.0f);
+      const float x4[4] = {u[0][t] * inv[t], u[1][t] * inv[t], u[2][t] * inv[t],
+                           u[3][t] * inv[t]};
+      uint32_t* dst = (uint32_t*)(U + (uint64_t)t * C32p * Kp * 32u + base);
+      if (PLANES == 2) {
+        uint32_t hi, lo;
+        e4m3x4_split(x4, hi, lo);
+        dst[0] = hi;
+        dst[plane / 4] = lo;
+      } else {
+        dst[0] = e4m3x4(x4[0], x4[1], x4[2], x4[3]);
+      }
     }
+  }
+}
+
+// Test hook: the kernels' e4m3 quantizer applied to n floats.
+__global__ void k_quantize_e4m3(const float* __restrict__ x, uint8_t* __restrict__ q, uint64_t n) {
+  const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4u;
+  if (i + 4u <= n) {
+    *(uint32_t*)(q + i) = e4m3x4(x[i], x[i + 1], x[i + 2], x[i + 3]);
+  } else if (i < n) {
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    for (uint64_t j = i; j < n; ++j) v[j - i] = x[j];
+    const uint32_t w = e4m3x4(v[0], v[1], v[2], v[3]);
+    for (uint64_t j = i; j < n; ++j) q[j] = (uint8_t)(w >> (8 * (j - i)));
   }
 }
 
@@ -217,6 +242,12 @@ hipError_t launch_input_amax(const _Float16* x, uint64_t count, float* amax, hip
     hipLaunchKernelGGL(k_amax<true>, dim3(blocks), dim3(256), 0, st, x, count, amax);
   else
     hipLaunchKernelGGL(k_amax<false>, dim3(blocks), dim3(256), 0, st, x, count, amax);
+  return hipGetLastError();
+}
+
+hipError_t launch_quantize_e4m3(const float* x, uint8_t* q, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quantize_e4m3, dim3(div_up((n + 3) / 4, 256)), dim3(256), 0, st, x, q, n);
   return hipGetLastError();
 }
 

@@ -177,6 +177,34 @@ __device__ inline float e4m3_to_f32(uint32_t b) {
   return (b & 0x80u) ? -v : v;
 }
 
+// The same quantizer on the hardware converter (v_cvt_pk_fp8_f32, OCP
+// e4m3 on gfx950, round to nearest even), four values per call, byte i =
+// value i.  The clamp reproduces the saturation of f32_to_e4m3; the kernels'
+// power-of-two scales keep every operand inside +-448 already.
+__device__ inline uint32_t e4m3x4(float a, float b, float c, float d) {
+  a = __builtin_amdgcn_fmed3f(a, 448.0f, -448.0f);
+  b = __builtin_amdgcn_fmed3f(b, 448.0f, -448.0f);
+  c = __builtin_amdgcn_fmed3f(c, 448.0f, -448.0f);
+  d = __builtin_amdgcn_fmed3f(d, 448.0f, -448.0f);
+  int r = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  r = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, r, true);
+  return (uint32_t)r;
+}
+
+// Value of byte SEL of a packed e4m3 word (hardware converter, exact).
+template <int SEL>
+__device__ inline float e4m3_at(uint32_t w) {
+  return __builtin_amdgcn_cvt_f32_fp8((int)w, SEL);
+}
+
+// hi = e4m3(x), lo = e4m3((x - hi) * 16) for four values: the two planes of
+// the FP8X2 split.
+__device__ inline void e4m3x4_split(const float (&x)[4], uint32_t& hi, uint32_t& lo) {
+  hi = e4m3x4(x[0], x[1], x[2], x[3]);
+  lo = e4m3x4((x[0] - e4m3_at<0>(hi)) * 16.0f, (x[1] - e4m3_at<1>(hi)) * 16.0f,
+              (x[2] - e4m3_at<2>(hi)) * 16.0f, (x[3] - e4m3_at<3>(hi)) * 16.0f);
+}
+
 // Smallest power of two s with amax / s <= 448 (1 for amax == 0).
 __host__ __device__ inline float pow2_scale(float amax) {
   if (!(amax > 0.0f)) return 1.0f;

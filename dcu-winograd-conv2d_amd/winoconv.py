@@ -90,6 +90,7 @@ _SIGS = {
     "wino_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
     "wino_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
     "wino_device_synchronize": (C.c_int, []),
+    "wino_quantize_e4m3": (C.c_int, [_P, _P, C.c_uint64]),
     "wino_engine_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_size_t]),
     "wino_engine_destroy": (C.c_int, [_P]),
     "wino_engine_load_tuning": (C.c_int, [_P, C.c_char_p]),

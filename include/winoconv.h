@@ -200,6 +200,13 @@ int wino_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int wino_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int wino_device_synchronize(void);
 
+/* Test hook: applies the e4m3 quantizer of the fp8 transform kernels (OCP
+ * e4m3, round to nearest even, saturating at +-448) to n floats at device
+ * address x, writing n bytes at device address q; synchronous.  Lets a host
+ * check the device quantizer against its own.  No counterpart in the
+ * reference. */
+int wino_quantize_e4m3(const void* x, void* q, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif
