@@ -15,20 +15,51 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 
 // ------------------------------------------------------------------ input
 //
-// A block stages CG channels of a window of tiles (plus the 2-pixel halo,
-// zero outside the image) into LDS: thread -> (row slot, column), rows
-// advance incrementally, so a wave reads contiguous runs of input rows.
-// Then each thread transforms one tile for CPT channels (2 for fp16, 4 for
-// fp8) and stores CPT consecutive channel values at once: a wave writes
-// 256 contiguous bytes of V per tile position.
+// A block stages CG channels of a window of WR x WC tiles plus the 2-pixel
+// halo into LDS, then transforms every tile of the window.
+//
+// Staging: LDS column j of a staged row holds input column xa + j with
+// xa = (first tile column) * m - 2, an even column, so with an even image
+// width every input dword (2 halves) lands on an LDS dword.  Thread ->
+// (row slot, dword column); each thread issues SB independent buffer loads
+// before it writes any of them to LDS, so SB loads per thread are in flight
+// together.  Pixels outside the image and channels past C read as zero.
+//
+// Transform: each thread handles one tile for CPT channels (2 for fp16, 4
+// for fp8).  Per tile row it reads the m+2 halves it needs as 8-byte (m = 4)
+// or 4-byte (m = 2) aligned LDS words, transforms in FP32 and stores CPT
+// consecutive channel values per tile position at once (a wave writes
+// contiguous runs of V).
 // PREC: 0 fp16 (CG = 8), 1 fp8, 2 fp8 hi/lo (CG = 32).
-template <int m, int CG, int PREC>
+
+// LDS geometry of one staged window, in halves: rows, row pitch (>= ntw*m+3
+// columns, a multiple of 4), plane pitch (in dwords = 8 mod 32, so the
+// channel lanes of one tile read different banks).  in_lds_bytes bounds it
+// for every window not larger than nth x ntw.
+struct InLds {
+  int LH, LWp, PS;
+};
+__host__ __device__ inline InLds in_lds_geom(int m, uint32_t nth, uint32_t ntw) {
+  InLds L;
+  L.LH = (int)nth * m + 2;
+  L.LWp = ((int)ntw * m + 3 + 3) & ~3;
+  const int psd = L.LH * L.LWp / 2;
+  L.PS = 2 * (psd + (((8 - psd) % 32) + 32) % 32);
+  return L;
+}
+__host__ __device__ inline uint32_t in_lds_bytes(int m, int CG, uint32_t nth, uint32_t ntw) {
+  const InLds L = in_lds_geom(m, nth, ntw);
+  return (uint32_t)CG * (uint32_t)(L.LH * L.LWp / 2 + 31) * 4u;
+}
+
+template <int m, int CG, int PREC, bool EVENW>
 __device__ inline void input_block(const half_t* __restrict__ X, void* __restrict__ Vout,
                                    const InputGeom& g, const float* __restrict__ amax,
                                    float* __restrict__ vscale, uint32_t bx, uint32_t cg,
                                    half_t* xs, bool write_scale) {
   constexpr int T = m + 2, T2 = T * T;
   constexpr int CPT = PREC == 0 ? 2 : 4;  // channels per thread in the transform
+  constexpr int SB = 8;                    // loads in flight per thread
   uint32_t b = bx;
   const uint32_t wc = b % g.nwc;
   b /= g.nwc;
@@ -36,32 +67,59 @@ __device__ inline void input_block(const half_t* __restrict__ X, void* __restric
   const uint32_t n = b / g.nwr;
   const uint32_t th0 = wr * g.WR, tw0 = wc * g.WC;
   const uint32_t nth = min(g.WR, g.TH - th0), ntw = min(g.WC, g.TW - tw0);
-  const int LH = (int)nth * m + 2, LW = (int)ntw * m + 2;
-  const int y0 = (int)(th0 * m) - 1, x0 = (int)(tw0 * m) - 1;
+  const InLds L = in_lds_geom(m, nth, ntw);
+  const int y0 = (int)(th0 * m) - 1, xa = (int)(tw0 * m) - 2;
   const int tid = threadIdx.x;
 
-  // stage the window: rows (channel-major) x columns
+  // ---- stage
   {
-    const int rpi = 256 / LW;  // rows per pass (LW <= 256 by construction)
-    const int col = tid % LW;
-    int rr = tid / LW;
-    int ch = rr / LH, lr = rr - ch * LH;
-    const int x = x0 + col;
-    const bool xin = x >= 0 && x < (int)g.W;
-    const half_t* plane0 = X + ((uint64_t)n * g.C + cg * CG) * g.H * g.W;
-    if (rr < rpi) {
-      for (; ch < CG; rr += rpi) {
-        const int y = y0 + lr;
-        const uint32_t c = cg * CG + ch;
-        half_t v = (half_t)0.0f;
-        if (xin && c < g.C && y >= 0 && y < (int)g.H)
-          v = plane0[((uint64_t)ch * g.H + y) * g.W + x];
-        xs[(ch * LH + lr) * LW + col] = v;
-        lr += rpi;
-        while (lr >= LH) {
-          lr -= LH;
-          ++ch;
+    const int LWd = L.LWp / 2;  // dwords per LDS row (<= 256 by construction)
+    const int rpp = 256 / LWd;  // rows per pass
+    const int dcol = tid % LWd, slot = tid / LWd;
+    const int x = xa + 2 * dcol;
+    const bool x0in = x >= 0 && x < (int)g.W, x1in = x + 1 >= 0 && x + 1 < (int)g.W;
+    const uint32_t c0 = cg * CG;
+    const uint32_t nch = c0 < g.C ? min((uint32_t)CG, g.C - c0) : 0u;
+    const uint64_t plane = (uint64_t)g.H * g.W;
+    // byte range of the group's real channels: < 2^32 (tensors hold < 2^31 elements)
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(X + ((uint64_t)n * g.C + (nch ? c0 : 0u)) * plane), 0,
+        (int)(uint32_t)(nch * plane * 2u), 0x00020000);
+    uint32_t* xs32 = (uint32_t*)xs;
+    const int R = CG * L.LH;
+    const int dch = rpp / L.LH, dlr = rpp % L.LH;
+    if (slot < rpp) {
+      int r = slot, ch = slot / L.LH, lr = slot % L.LH;
+      while (r < R) {
+        uint32_t val[SB];
+        int dst[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int y = y0 + lr;
+          const bool iny = r < R && (uint32_t)ch < nch && y >= 0 && y < (int)g.H;
+          const uint32_t e = (uint32_t)(((uint64_t)ch * plane + (uint64_t)(y < 0 ? 0 : y) * g.W) * 2u);
+          if constexpr (EVENW) {
+            const bool in = iny && x0in;
+            const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rs, in ? e + 2u * x : 0u, 0, 0);
+            val[u] = in ? v : 0u;
+          } else {
+            const bool in0 = iny && x0in, in1 = iny && x1in;
+            const uint32_t a = __builtin_amdgcn_raw_buffer_load_b16(rs, in0 ? e + 2u * x : 0u, 0, 0);
+            const uint32_t c = __builtin_amdgcn_raw_buffer_load_b16(rs, in1 ? e + 2u * x + 2u : 0u, 0, 0);
+            val[u] = (in0 ? (a & 0xffffu) : 0u) | ((in1 ? (c & 0xffffu) : 0u) << 16);
+          }
+          dst[u] = r < R ? (ch * L.PS + lr * L.LWp) / 2 + dcol : -1;
+          r += rpp;
+          ch += dch;
+          lr += dlr;
+          if (lr >= L.LH) {
+            lr -= L.LH;
+            ++ch;
+          }
         }
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+          if (dst[u] >= 0) xs32[dst[u]] = val[u];
       }
     }
   }
@@ -78,49 +136,64 @@ __device__ inline void input_block(const half_t* __restrict__ X, void* __restric
     }
   }
 
+  // ---- transform
   constexpr int LANES_PER_TILE = CG / CPT;  // 4 (fp16) or 8 (fp8)
-  const int cq = tid % LANES_PER_TILE;       // channel quad/pair index
-  const int per_ch = LH * LW;
+  const int cq = tid % LANES_PER_TILE;
   const int ntiles = (int)(nth * ntw);
   const uint64_t tplane = (uint64_t)g.CGp * g.Pp * CG;  // elements per tile position
   for (int i = tid / LANES_PER_TILE; i < ntiles; i += 256 / LANES_PER_TILE) {
     const int lth = i / (int)ntw, ltw = i - lth * (int)ntw;
     const uint64_t p = ((uint64_t)n * g.TH + th0 + lth) * g.TW + tw0 + ltw;
     const uint64_t off = ((uint64_t)cg * g.Pp + p) * CG + cq * CPT;
-    if (PREC == 0) {
-      half2_t out[T2];
+    float v[CPT][T2];
 #pragma unroll
-      for (int cc = 0; cc < 2; ++cc) {
-        const half_t* src = xs + (cq * 2 + cc) * per_ch + (lth * m) * LW + ltw * m;
-        float d[T2], v[T2];
+    for (int cc = 0; cc < CPT; ++cc) {
+      // LDS column ltw*m holds input column (tw0+ltw)*m - 2; the tile needs
+      // the m+2 columns after it
+      const half_t* src = xs + (cq * CPT + cc) * L.PS + (lth * m) * L.LWp + ltw * m;
+      float d[T2];
 #pragma unroll
-        for (int a = 0; a < T; ++a)
+      for (int a = 0; a < T; ++a) {
+        half_t h[m == 4 ? 8 : 6];
+        if constexpr (m == 4) {
+          const uint2* q = (const uint2*)(src + a * L.LWp);
+          const uint2 w0 = q[0], w1 = q[1];
+          const uint32_t ws[4] = {w0.x, w0.y, w1.x, w1.y};
 #pragma unroll
-          for (int c2 = 0; c2 < T; ++c2) d[a * T + c2] = (float)src[a * LW + c2];
-        input2d<m>(d, v);
+          for (int k = 0; k < 4; ++k) {
+            const half2_t hv = __builtin_bit_cast(half2_t, ws[k]);
+            h[2 * k] = hv[0];
+            h[2 * k + 1] = hv[1];
+          }
+        } else {
+          const uint32_t* q = (const uint32_t*)(src + a * L.LWp);
 #pragma unroll
-        for (int t = 0; t < T2; ++t) out[t][cc] = (half_t)v[t];
+          for (int k = 0; k < 3; ++k) {
+            const half2_t hv = __builtin_bit_cast(half2_t, q[k]);
+            h[2 * k] = hv[0];
+            h[2 * k + 1] = hv[1];
+          }
+        }
+#pragma unroll
+        for (int c2 = 0; c2 < T; ++c2) d[a * T + c2] = (float)h[c2 + 1];
       }
+      input2d<m>(d, v[cc]);
+    }
+    if (PREC == 0) {
       half_t* V = (half_t*)Vout;
 #pragma unroll
-      for (int t = 0; t < T2; ++t) *(half2_t*)(V + t * tplane + off) = out[t];
-    } else {
-      float vq[4][T2];
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc) {
-        const half_t* src = xs + (cq * 4 + cc) * per_ch + (lth * m) * LW + ltw * m;
-        float d[T2];
-#pragma unroll
-        for (int a = 0; a < T; ++a)
-#pragma unroll
-          for (int c2 = 0; c2 < T; ++c2) d[a * T + c2] = (float)src[a * LW + c2];
-        input2d<m>(d, vq[cc]);
+      for (int t = 0; t < T2; ++t) {
+        half2_t o;
+        o[0] = (half_t)v[0][t];
+        o[1] = (half_t)v[1][t];
+        *(half2_t*)(V + t * tplane + off) = o;
       }
+    } else {
       uint8_t* V = (uint8_t*)Vout;
 #pragma unroll
       for (int t = 0; t < T2; ++t) {
-        const float x4[4] = {vq[0][t] * inv[t], vq[1][t] * inv[t], vq[2][t] * inv[t],
-                             vq[3][t] * inv[t]};
+        const float x4[4] = {v[0][t] * inv[t], v[1][t] * inv[t], v[2][t] * inv[t],
+                             v[3][t] * inv[t]};
         if (PREC == 2) {
           uint32_t hi, lo;
           e4m3x4_split(x4, hi, lo);
@@ -134,13 +207,13 @@ __device__ inline void input_block(const half_t* __restrict__ X, void* __restric
   }
 }
 
-template <int m, int CG, int PREC>
+template <int m, int CG, int PREC, bool EVENW>
 __global__ __launch_bounds__(256) void k_input(const half_t* __restrict__ X, void* __restrict__ Vout,
                                                InputGeom g, const float* __restrict__ amax,
                                                float* __restrict__ vscale) {
-  extern __shared__ half_t xs_dyn[];
-  input_block<m, CG, PREC>(X, Vout, g, amax, vscale, blockIdx.x, blockIdx.y, xs_dyn,
-                           blockIdx.x == 0 && blockIdx.y == 0);
+  extern __shared__ __attribute__((aligned(16))) half_t xs_dyn[];
+  input_block<m, CG, PREC, EVENW>(X, Vout, g, amax, vscale, blockIdx.x, blockIdx.y, xs_dyn,
+                                  blockIdx.x == 0 && blockIdx.y == 0);
 }
 
 // ----------------------------------------------------------------- output
@@ -229,24 +302,18 @@ __global__ __launch_bounds__(256) void k_output(const half_t* __restrict__ M,
 // --------------------------------------------------------------- geometry
 
 void input_window(const Plan& pl, int CG, uint64_t budget, uint32_t& WR, uint32_t& WC) {
+  // widest window row first (whole image rows when they fit), then as many
+  // tile rows as the LDS budget allows, capped at ~4 transform passes of the
+  // block (64 tiles per pass for fp16, 32 for fp8) and at the image
   WC = pl.tiles_w < 32u ? pl.tiles_w : 32u;
-  for (;;) {
-    const uint64_t lw = (uint64_t)WC * pl.m + 2;
-    const uint64_t rows = budget / ((uint64_t)CG * 2u * lw);
-    if ((rows >= (uint64_t)pl.m + 2 && lw <= 256) || WC == 1) {
-      uint64_t wr = rows > 2 ? (rows - 2) / pl.m : 1;
-      if (wr < 1) wr = 1;
-      // enough tiles for ~4 passes of the block (64 tiles per pass for fp16,
-      // 32 for fp8), no more than the image
-      const uint64_t tpp = CG == 8 ? 64 : 32;
-      const uint64_t want = (4 * tpp + WC - 1) / WC;
-      if (wr > want) wr = want;
-      if (wr > pl.tiles_h) wr = pl.tiles_h;
-      WR = (uint32_t)(wr < 1 ? 1 : wr);
-      return;
-    }
+  while (WC > 1 && (in_lds_bytes(pl.m, CG, 1, WC) > budget || (WC * pl.m + 6) / 2 > 256))
     WC = (WC + 1) / 2;
-  }
+  const uint32_t tpp = CG == 8 ? 64u : 32u;
+  uint32_t wr = (4u * tpp + WC - 1) / WC;
+  if (wr > pl.tiles_h) wr = pl.tiles_h;
+  if (wr < 1) wr = 1;
+  while (wr > 1 && in_lds_bytes(pl.m, CG, wr, WC) > budget) --wr;
+  WR = wr;
 }
 
 InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_t W,
@@ -269,7 +336,7 @@ InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_
 
 static size_t input_lds(const Plan& pl, const InputGeom& g) {
   const int CG = pl.precision == WINO_PREC_FP16 ? 8 : 32;
-  return (size_t)CG * (g.WR * pl.m + 2) * (g.WC * pl.m + 2) * sizeof(half_t);
+  return in_lds_bytes(pl.m, CG, g.WR, g.WC);
 }
 
 // ------------------------------------------------------ single problem
@@ -281,22 +348,26 @@ hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N,
   const int CG = pl.precision == WINO_PREC_FP16 ? 8 : 32;
   const size_t lds = input_lds(pl, g);
   dim3 grid((uint32_t)((uint64_t)N * g.nwr * g.nwc), div_up(C, CG));
+#define WINO_IN(M, G, P)                                                                      \
+  do {                                                                                        \
+    if (W % 2 == 0)                                                                           \
+      hipLaunchKernelGGL((k_input<M, G, P, true>), grid, dim3(256), lds, st, x, V, g, amax,   \
+                         v_scale);                                                            \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_input<M, G, P, false>), grid, dim3(256), lds, st, x, V, g, amax,  \
+                         v_scale);                                                            \
+  } while (0)
   if (pl.precision == WINO_PREC_FP16) {
-    if (pl.m == 2)
-      hipLaunchKernelGGL((k_input<2, 8, 0>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-    else
-      hipLaunchKernelGGL((k_input<4, 8, 0>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
+    if (pl.m == 2) WINO_IN(2, 8, 0);
+    else WINO_IN(4, 8, 0);
   } else if (pl.precision == WINO_PREC_FP8) {
-    if (pl.m == 2)
-      hipLaunchKernelGGL((k_input<2, 32, 1>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-    else
-      hipLaunchKernelGGL((k_input<4, 32, 1>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
+    if (pl.m == 2) WINO_IN(2, 32, 1);
+    else WINO_IN(4, 32, 1);
   } else {
-    if (pl.m == 2)
-      hipLaunchKernelGGL((k_input<2, 32, 2>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
-    else
-      hipLaunchKernelGGL((k_input<4, 32, 2>), grid, dim3(256), lds, st, x, V, g, amax, v_scale);
+    if (pl.m == 2) WINO_IN(2, 32, 2);
+    else WINO_IN(4, 32, 2);
   }
+#undef WINO_IN
   return hipGetLastError();
 }
 
@@ -320,7 +391,7 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 template <int PREC>
 __global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __restrict__ hdr,
                                                        const GProb* __restrict__ probs) {
-  extern __shared__ half_t gxs[];
+  extern __shared__ __attribute__((aligned(16))) half_t gxs[];
   constexpr int CG = PREC == 0 ? 8 : 32;
   const uint32_t total = hdr->in_blocks, np = hdr->nprob;
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
@@ -329,10 +400,18 @@ __global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __rest
     const GProb& pr = probs[p];
     const uint32_t local = v - pr.in_first;
     const uint32_t bx = local % pr.in_gx, cg = local / pr.in_gx;
-    if (pr.m == 2)
-      input_block<2, CG, PREC>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
-    else
-      input_block<4, CG, PREC>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+    const bool evenw = (pr.ig.W & 1u) == 0;
+    if (pr.m == 2) {
+      if (evenw)
+        input_block<2, CG, PREC, true>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+      else
+        input_block<2, CG, PREC, false>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+    } else {
+      if (evenw)
+        input_block<4, CG, PREC, true>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+      else
+        input_block<4, CG, PREC, false>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+    }
     __syncthreads();  // xs is restaged by the next virtual block
   }
 }
