@@ -1,13 +1,26 @@
-"""Throughput bench of the MI355X Winograd 3x3 conv2d operator (one GPU).
+"""Throughput / latency bench of the MI355X Winograd 3x3 conv2d operator (one GPU).
 
-Workload: N=64, C=256, H=W=28, K=256, 3x3 stride 1 pad 1, NCHW fp16 (input
-N(0,1) clipped to +-4, He-init filters, seeded, synthetic).  One step = one
-forward convolution of the whole batch with inputs resident in HBM.
+Everything runs on synthetic, seeded data of the configured shapes, with the
+inputs resident in HBM before any timed region:
 
-Metric: effective conv TFLOP/s = 2*N*K*Oh*Ow*C*9 / step time, for the chosen
-path (the reference selector's algorithm, F(2x3) fused here, with the filter
-transform kept resident) beside the plain FP16 non-fused Winograd path on the
-same shape.  Prints one JSON line.
+  conv    N=64, C=256, H=W=28, K=256, 3x3 s1 p1 NCHW fp16; input N(0,1)
+          clipped to +-4, He-init filter.  One step = one forward conv of the
+          whole batch.  Paths: plain FP16 non-fused F(2x3) and F(4x3) (filter
+          transform on every call, as the reference), and the library's
+          resident-filter paths (F(4x3) non-fused, F(2x3) fused, fp8x2).  The
+          chosen path is the fastest resident path whose relative L2 error vs
+          an FP64 direct convolution (torch, CPU, two images) is <= 1e-2.
+  fp8     N=32, C=512, H=W=14, K=512 F(4x3): fp16 vs fp8x2 vs fp8 GEMM.
+  grouped 16 uneven problems, (C,K,H=W) from {(64,64,56), (128,128,28),
+          (256,256,14), (512,512,7)}, N uniform in 1..32: one hipGraph-replayed
+          grouped launch vs the plain per-problem FP16 launches.
+  serving Poisson arrivals at 3 rates (fractions of the measured capacity),
+          N uniform in 1..16 over the same shape set, resident filters,
+          admission capped by the workspace pool; p50/p99 latency; 1% of the
+          results checked against FP64.
+
+Prints one JSON line (value = effective TFLOP/s of the chosen path on the
+conv workload; effective = direct-conv FLOPs 2*N*K*H*W*C*9 / time).
 """
 import argparse
 import json
@@ -25,147 +38,415 @@ import numpy as np  # noqa: E402
 PEAK_HBM = 8.0e12           # B/s, MI355X
 PEAK_MFMA = {"fp16": 2.5e15, "fp8": 5.0e15}
 N, C, H, W, K = 64, 256, 28, 28, 256
+CLASSES = [(64, 64, 56), (128, 128, 28), (256, 256, 14), (512, 512, 7)]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def inputs(seed=2026):
-    rng = np.random.default_rng(seed)
-    x = np.clip(rng.standard_normal((N, C, H, W)), -4, 4).astype(np.float16)
-    wt = (rng.standard_normal((K, C, 3, 3)) * np.sqrt(2.0 / (9 * C))).astype(np.float16)
-    return x, wt
+def he_inputs(rng, n, c, h, w, k, clip=4.0):
+    x = rng.standard_normal((n, c, h, w))
+    if clip:
+        x = np.clip(x, -clip, clip)
+    wt = rng.standard_normal((k, c, 3, 3)) * np.sqrt(2.0 / (9 * c))
+    return x.astype(np.float16), wt.astype(np.float16)
 
 
-def direct_flops(n=N):
-    return 2.0 * n * K * H * W * C * 9
+def flops(n, c, h, w, k):
+    return 2.0 * n * k * h * w * c * 9
 
 
-def time_steps(torch, fn, steps, warmup):
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int16)).view(torch.float16).cuda()
+
+
+def host(torch, t):
+    torch.cuda.synchronize()
+    return t.cpu().view(torch.int16).numpy().view(np.float16)
+
+
+def ref_f64(torch, x, wt, images):
+    """FP64 direct convolution of the chosen images (torch on the CPU)."""
+    xs = torch.from_numpy(x[images].astype(np.float64))
+    ws = torch.from_numpy(wt.astype(np.float64))
+    return torch.nn.functional.conv2d(xs, ws, padding=1).numpy()
+
+
+def rel_l2(got, ref):
+    got = got.astype(np.float64)
+    return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def time_loop(torch, fn, steps, warmup):
+    """Mean ms per call over `steps` calls on the null stream (HIP events)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
     for _ in range(steps):
         fn()
-    stop.record()
+    b.record()
     torch.cuda.synchronize()
-    return start.elapsed_time(stop) / steps
+    return a.elapsed_time(b) / steps
 
 
-def latencies(torch, fn, reps=50):
-    out = []
-    for _ in range(reps):
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        fn()
-        b.record()
-        b.synchronize()
-        out.append(a.elapsed_time(b))
-    return np.array(out)
+def load_traffic(kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("kernels", {}).get(kernel)
+    if not e:
+        return None
+    return float(e["bytes_per_dispatch"])
 
 
-def cpu_baseline(x, wt):
-    """The reference's CPU direct conv (verfiy.h, built from the reference
-    tree into oracle/_ref) on a bounded sample: images 0..S-1, timed on one
-    host core; falls back to the oracle's C restatement."""
+# ------------------------------------------------------------------ conv
+def bench_conv(torch, wc, args):
+    rng = np.random.default_rng(2026)
+    x, wt = he_inputs(rng, N, C, H, W, K)
+    dx, dw = dev(torch, x), dev(torch, wt)
+    dy = torch.empty((N, K, H, W), dtype=torch.float16, device="cuda")
+    images = [0, N - 1]
+    ref = ref_f64(torch, x, wt, images)
+    F = flops(N, C, H, W, K)
+    paths = [("plain_f23", wc.ALGO_F23, wc.PREC_FP16, False),
+             ("plain_f43", wc.ALGO_F43, wc.PREC_FP16, False),
+             ("f43", wc.ALGO_F43, wc.PREC_FP16, True),
+             ("fused_f23", wc.ALGO_F23_FUSED, wc.PREC_FP16, True),
+             ("f23", wc.ALGO_F23, wc.PREC_FP16, True),
+             ("f43_fp8x2", wc.ALGO_F43, wc.PREC_FP8X2, True)]
+    if args.only:
+        paths = [p for p in paths if p[0] == args.only]
+    res = {}
+    for name, algo, prec, resident in paths:
+        op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), N, C, H, W, K, algo=algo,
+                       precision=prec, resident=resident)
+        if resident:
+            op.prepare_filter()
+        ms = time_loop(torch, op, args.steps, args.warmup)
+        y = host(torch, dy)
+        err = rel_l2(y[images], ref)
+        res[name] = {"us": ms * 1e3, "tflops": F / (ms * 1e-3) / 1e12, "rel_l2": err,
+                     "resident": resident, "op": op}
+        log(f"conv {name}: {ms * 1e3:.1f} us, {res[name]['tflops']:.1f} eff TFLOP/s, "
+            f"rel-L2 {err:.2e}")
+    return res, F
+
+
+def dominant_stage(torch, wc, op, args):
+    """Times each non-fused stage of `op` alone (HIP events on the null
+    stream, the stream the stages are launched on); returns the slowest."""
+    inf = op.info()
+    stages = ["input", "gemm", "output"]
+    out = {}
+    for s in stages:
+        out[s] = time_loop(torch, lambda s=s: op.stage(s), args.steps, args.warmup) * 1e3
+    return out, inf
+
+
+# ------------------------------------------------------------------ fp8
+def bench_fp8(torch, wc, args):
+    n, c, h, w, k = 32, 512, 14, 14, 512
+    rng = np.random.default_rng(2027)
+    x, wt = he_inputs(rng, n, c, h, w, k, clip=None)
+    dx, dw = dev(torch, x), dev(torch, wt)
+    dy = torch.empty((n, k, h, w), dtype=torch.float16, device="cuda")
+    images = [0, n - 1]
+    ref = ref_f64(torch, x, wt, images)
+    F = flops(n, c, h, w, k)
+    out = {}
+    for name, prec in [("fp16", wc.PREC_FP16), ("fp8x2", wc.PREC_FP8X2), ("fp8", wc.PREC_FP8)]:
+        op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k,
+                       algo=wc.ALGO_F43, precision=prec, resident=True)
+        op.prepare_filter()
+        ms = time_loop(torch, op, args.steps, args.warmup)
+        y = host(torch, dy)
+        out[name] = {"us": round(ms * 1e3, 2), "eff_tflops": round(F / (ms * 1e-3) / 1e12, 2),
+                     "rel_l2": float(f"{rel_l2(y[images], ref):.3e}"),
+                     "max_abs": float(f"{np.abs(y[images].astype(np.float64) - ref).max():.3e}")}
+        op.destroy()
+        log(f"fp8 workload F(4x3) {name}: {out[name]}")
+    return out
+
+
+# ------------------------------------------------------------------ grouped
+def make_problems(rng, count, nmax):
+    probs = []
+    for _ in range(count):
+        c, k, hw = CLASSES[int(rng.integers(len(CLASSES)))]
+        probs.append((int(rng.integers(1, nmax + 1)), c, hw, hw, k))
+    return probs
+
+
+def bench_grouped(torch, wc, args, tuning):
+    rng = np.random.default_rng(2028)
+    probs = make_problems(rng, 16, 32)
+    data = []
+    for (n, c, h, w, k) in probs:
+        x, wt = he_inputs(rng, n, c, h, w, k)
+        data.append((x, wt, dev(torch, x), dev(torch, wt),
+                     torch.empty((n, k, h, w), dtype=torch.float16, device="cuda")))
+    F = sum(flops(*p[:4], p[4]) for p in probs)
+    torch.cuda.synchronize()
+    eng = wc.Engine(wc.PREC_FP16, 2 << 30, tuning=tuning)
+    batch = wc.Batch(eng, 16)
+    stream = torch.cuda.Stream()
+    try:
+        fids = [eng.register_filter(d[3].data_ptr(), p[4], p[1]) for p, d in zip(probs, data)]
+        for (n, c, h, w, k), fid, d in zip(probs, fids, data):
+            batch.add(fid, d[2].data_ptr(), d[4].data_ptr(), n, h, w)
+        algos = [batch.info(i).m for i in range(len(probs))]
+        batch.capture(stream.cuda_stream)
+        with torch.cuda.stream(stream):
+            ms = time_loop(torch, lambda: batch.launch(stream.cuda_stream), args.steps, args.warmup)
+        errs = []
+        for i in (0, 7, 15):
+            x, wt = data[i][0], data[i][1]
+            errs.append(rel_l2(host(torch, data[i][4])[:1], ref_f64(torch, x, wt, [0])))
+        # plain: the same problems one by one, FP16 non-fused, filter
+        # transformed on every call, same per-problem algorithm
+        ops = []
+        for (n, c, h, w, k), d, m in zip(probs, data, algos):
+            ops.append(wc.Conv2d(d[2].data_ptr(), d[3].data_ptr(), d[4].data_ptr(), n, c, h, w, k,
+                                 algo=wc.ALGO_F43 if m == 4 else wc.ALGO_F23))
+
+        def plain():
+            for op in ops:
+                op()
+        ms_plain = time_loop(torch, plain, args.steps, args.warmup)
+        for op in ops:
+            op.destroy()
+    finally:
+        stream.synchronize()
+        batch.destroy()
+        eng.destroy()
+    out = {"problems": [list(p) for p in probs], "gflop": round(F / 1e9, 2),
+           "algo_m": algos, "us": round(ms * 1e3, 2),
+           "eff_tflops": round(F / (ms * 1e-3) / 1e12, 2),
+           "plain_us": round(ms_plain * 1e3, 2),
+           "plain_eff_tflops": round(F / (ms_plain * 1e-3) / 1e12, 2),
+           "grouped_over_plain": round(ms_plain / ms, 3),
+           "rel_l2_sampled": [float(f"{e:.3e}") for e in errs], "hipgraph": True,
+           "tuning_table": os.path.relpath(tuning, ROOT) if tuning else None}
+    log(f"grouped: {out}")
+    return out
+
+
+# ------------------------------------------------------------------ serving
+def bench_serving(torch, wc, args, tuning):
+    rng = np.random.default_rng(2029)
+    nmax = 16
+    eng = wc.Engine(wc.PREC_FP16, 4 << 30, tuning=tuning)
+    out = {"rates": []}
+    try:
+        cls = []
+        for (c, k, hw) in CLASSES:
+            x, wt = he_inputs(rng, nmax, c, hw, hw, k)
+            dx, dw = dev(torch, x), dev(torch, wt)
+            fid = eng.register_filter(dw.data_ptr(), k, c)
+            ring = [torch.empty((nmax, k, hw, hw), dtype=torch.float16, device="cuda")
+                    for _ in range(8)]
+            cls.append({"x": x, "wt": wt, "dx": dx, "dw": dw, "fid": fid, "ring": ring, "c": c,
+                        "k": k, "hw": hw})
+        torch.cuda.synchronize()
+        nreq = args.requests
+        capacity = None
+        # first pass: every request offered at once (saturation) measures the
+        # capacity; then Poisson arrivals at 30/60/90% of it
+        for frac in (None, 0.3, 0.6, 0.9):
+            qps = 1e12 if frac is None else frac * capacity
+            srv = wc.Server(eng, slots=4, max_batch=16, use_graph=True)
+            reqs, checks = [], []
+            F = 0.0
+            try:
+                t = time.perf_counter()
+                t0 = t
+                for r in range(nreq):
+                    ci = int(rng.integers(len(CLASSES)))
+                    n = int(rng.integers(1, nmax + 1))
+                    cl = cls[ci]
+                    sample = r % 100 == 0
+                    y = (torch.empty((n, cl["k"], cl["hw"], cl["hw"]), dtype=torch.float16,
+                                     device="cuda") if sample else cl["ring"][r % 8])
+                    if sample:
+                        torch.cuda.synchronize()
+                    while time.perf_counter() < t:
+                        pass
+                    rid = srv.submit(cl["fid"], cl["dx"].data_ptr(), y.data_ptr(), n, cl["hw"],
+                                     cl["hw"])
+                    reqs.append(rid)
+                    F += flops(n, cl["c"], cl["hw"], cl["hw"], cl["k"])
+                    if sample:
+                        checks.append((rid, ci, n, y))
+                    t += rng.exponential(1.0 / qps)
+                srv.drain()
+                wall = time.perf_counter() - t0
+                lat = np.array([srv.poll(rid) for rid in reqs])
+                st = srv.stats()
+            finally:
+                srv.destroy()
+            errs = []
+            for rid, ci, n, y in checks:
+                cl = cls[ci]
+                got = host(torch, y)[:1]
+                errs.append(rel_l2(got, ref_f64(torch, cl["x"][:n], cl["wt"], [0])))
+            row = {"offered_qps": round(qps, 1), "requests": nreq,
+                   "achieved_qps": round(nreq / wall, 1),
+                   "eff_tflops": round(F / wall / 1e12, 2),
+                   "p50_us": round(float(np.percentile(lat, 50)), 1),
+                   "p99_us": round(float(np.percentile(lat, 99)), 1),
+                   "batches": int(st.batches), "max_batch": int(st.max_batch_problems),
+                   "max_inflight_batches": int(st.max_inflight_batches),
+                   "pool_peak_mib": round(st.pool_peak / 2**20, 1),
+                   "checked": len(errs), "max_rel_l2": float(f"{max(errs):.3e}") if errs else None}
+            if frac is None:
+                capacity = nreq / wall
+                out["capacity_qps"] = round(capacity, 1)
+                out["saturated"] = row
+            else:
+                out["rates"].append(row)
+            log(f"serving: {row}")
+    finally:
+        eng.destroy()
+    return out
+
+
+def cpu_baseline(budget_s=12.0):
+    """The reference's CPU direct conv (verfiy.h compiled from the reference
+    tree into oracle/_ref; the oracle's C restatement when that is absent) on
+    whole images of the conv workload until ~budget_s of CPU work, one host
+    core."""
     from oracle import oracle as O
-    nimg = 1
-    xs = np.ascontiguousarray(x[:nimg])
+    rng = np.random.default_rng(2026)
+    x, wt = he_inputs(rng, N, C, H, W, K)
     kind = "reference" if O.ref_lib() is not None else "port"
     fn = O.conv2d_verfiy_ref if kind == "reference" else O.conv2d_verfiy
-    t0 = time.perf_counter()
-    fn(xs, wt)
+    done, t0 = 0, time.perf_counter()
+    while done < N and time.perf_counter() - t0 < budget_s:
+        fn(np.ascontiguousarray(x[done:done + 1]), wt)
+        done += 1
     dt = time.perf_counter() - t0
-    return {"value": direct_flops(nimg) / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"{nimg} of {N} images of the same workload ({direct_flops(nimg)/1e9:.2f} "
-                      f"GFLOP, {dt:.1f} s, verfiy.h single-threaded)"}
+    F = flops(done, C, H, W, K)
+    return {"value": round(F / dt / 1e12, 6), "unit": "TFLOP/s", "cores": 1, "kind": kind,
+            "sample": f"{done} of {N} images of the conv workload ({F / 1e9:.2f} GFLOP in "
+                      f"{dt:.1f} s, verfiy.h conv2dcpu, single-threaded)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default=None, help="run one conv path only (for profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the operator on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: this operator runs on a single MI355X; sharding "
+              "across GPUs is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
-
     import torch
+    if args.copy_calibration:
+        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel(), "copies": 5}))
+        return
     import winoconv as wc
 
-    x, wt = inputs()
-    dx = torch.from_numpy(x.view(np.int16)).view(torch.float16).cuda()
-    dw = torch.from_numpy(wt.view(np.int16)).view(torch.float16).cuda()
-    dy = torch.empty((N, K, H, W), dtype=torch.float16, device="cuda")
-    flops = direct_flops()
+    tuning = os.path.join(ROOT, "dcu-winograd-conv2d_amd", "tuning", "mi355x.tsv")
+    tuning = tuning if os.path.exists(tuning) else None
+    conv, F = bench_conv(torch, wc, args)
+    if args.only:
+        for v in conv.values():
+            v["op"].destroy()
+        return
+    resident = {k: v for k, v in conv.items() if v["resident"] and v["rel_l2"] <= 1e-2}
+    chosen_name = min(resident, key=lambda k: resident[k]["us"])
+    plain_name = min(("plain_f23", "plain_f43"), key=lambda k: conv[k]["us"])
+    chosen, plain = conv[chosen_name], conv[plain_name]
 
-    def mk(algo, prec=wc.PREC_FP16, resident=False):
-        return wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), N, C, H, W, K, algo=algo,
-                         precision=prec, resident=resident)
+    # roofline of the chosen path's dominant kernel, timed alone
+    op = chosen["op"]
+    inf = op.info()
+    T2 = inf.t2
+    P = inf.tiles
+    if chosen_name.startswith("fused"):
+        kern = "k_fused_f23"
+        kus = chosen["us"]
+        achieved = 2.0 * T2 * K * C * P / (kus * 1e-6)
+        roof = {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
+                "peak": PEAK_MFMA["fp16"] / 1e12, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_MFMA["fp16"], 4), "traffic": None}
+    else:
+        st_us, _ = dominant_stage(torch, wc, op, args)
+        log(f"stages of {chosen_name} alone (us): {st_us}")
+        dom = max(st_us, key=st_us.get)
+        b_op = 1 if inf.precision == wc.PREC_FP8 else 2  # bytes per U/V element
+        alg = {  # algorithmic HBM bytes per launch
+            "input": 2.0 * N * C * H * W + T2 * C * P * b_op,
+            "gemm": T2 * (K * C * b_op + C * P * b_op) + T2 * K * P * 2.0,
+            "output": T2 * K * P * 2.0 + 2.0 * N * K * H * W}
+        kern = {"input": "k_input", "gemm": "k_gemm", "output": "k_output"}[dom]
+        achieved = alg[dom] / (st_us[dom] * 1e-6)
+        traffic = load_traffic(f"{kern}:{chosen_name}")
+        roof = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
+                "traffic": traffic, "kernel": kern, "stage_us": {k: round(v, 2) for k, v in st_us.items()},
+                "algorithmic_bytes": alg[dom]}
+        if dom == "gemm":
+            gf = 2.0 * T2 * K * C * P / (st_us["gemm"] * 1e-6)
+            roof["mfma_tflops"] = round(gf / 1e12, 2)
+            roof["mfma_frac"] = round(gf / PEAK_MFMA["fp16"], 4)
+    for v in conv.values():
+        v["op"].destroy()
 
-    results = {}
-    paths = [("plain_f43", wc.ALGO_F43, wc.PREC_FP16, False),
-             ("plain_f23", wc.ALGO_F23, wc.PREC_FP16, False),
-             ("fused_f23", wc.ALGO_F23_FUSED, wc.PREC_FP16, False),
-             ("fused_f23_resident", wc.ALGO_F23_FUSED, wc.PREC_FP16, True),
-             ("f43_fp8x2", wc.ALGO_F43, wc.PREC_FP8X2, False)]
-    for name, algo, prec, res in paths:
-        op = mk(algo, prec, res)
-        if res:
-            op.prepare_filter()
-        ms = time_steps(torch, op, args.steps, args.warmup)
-        results[name] = {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12}
-        log(f"{name}: {ms * 1e3:.1f} us/step, {results[name]['tflops']:.1f} eff TFLOP/s")
-        if name == "fused_f23_resident":
-            lat = latencies(torch, op)
-            results[name]["p50_ms"] = float(np.percentile(lat, 50))
-            results[name]["p99_ms"] = float(np.percentile(lat, 99))
-        op.destroy()
+    fp8 = bench_fp8(torch, wc, args)
+    grouped = bench_grouped(torch, wc, args, tuning)
+    serving = bench_serving(torch, wc, args, tuning)
 
-    chosen = results["fused_f23_resident"]
-    plain = results["plain_f43"]
-    # dominant kernel of the chosen path: the fused kernel is its only launch;
-    # algorithmic MFMA work = 2 * 16 * K * C * P (P = N * 14 * 14 tiles)
-    P = N * ((H + 1) // 2) * ((W + 1) // 2)
-    gemm_flops = 2.0 * 16 * K * C * P
-    achieved = gemm_flops / (chosen["ms"] * 1e-3)
     line = {
-        "metric": "effective conv TFLOP/s (direct-conv FLOPs / step time), chosen vs plain FP16 non-fused Winograd",
+        "metric": "effective conv TFLOP/s + p99 latency @ fixed QPS, chosen vs plain Winograd, same err",
         "value": round(chosen["tflops"], 2),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(chosen["ms"], 5),
+        "ms_per_step": round(chosen["us"] / 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16",
-        "data": "synthetic (seeded N(0,1) clipped +-4 input, He-init filters)",
-        "config": {"workload": "conv2d 3x3 s1 p1 NCHW fp16 N=64 C=256 H=W=28 K=256",
-                   "chosen_path": "F(2x3) fused, resident pre-transformed filter",
-                   "plain_path": "F(4x3) non-fused FP16 (4 kernels, filter transform per call)",
-                   "plain_tflops": round(plain["tflops"], 2),
-                   "chosen_over_plain": round(chosen["tflops"] / plain["tflops"], 3),
-                   "p50_ms": round(chosen.get("p50_ms", 0.0), 4),
-                   "p99_ms": round(chosen.get("p99_ms", 0.0), 4),
-                   "paths": {k: {"us_per_step": round(v["ms"] * 1e3, 2),
-                                 "eff_tflops": round(v["tflops"], 2)} for k, v in results.items()}},
-        "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
-                     "peak": PEAK_MFMA["fp16"] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_MFMA["fp16"], 4), "traffic": None},
+        "dtype": "fp16" if "fp8" not in chosen_name else "fp8x2",
+        "data": "synthetic (seeded N(0,1) inputs clipped +-4, He-init filters)",
+        "config": {
+            "workload": "conv2d 3x3 s1 p1 NCHW fp16 N=64 C=256 H=W=28 K=256",
+            "chosen_path": chosen_name, "plain_path": plain_name,
+            "chosen_over_plain": round(plain["us"] / chosen["us"], 3),
+            "chosen_rel_l2": float(f"{chosen['rel_l2']:.3e}"),
+            "plain_rel_l2": float(f"{plain['rel_l2']:.3e}"),
+            "paths": {k: {"us": round(v["us"], 2), "eff_tflops": round(v["tflops"], 2),
+                          "rel_l2": float(f"{v['rel_l2']:.3e}"), "resident_filter": v["resident"]}
+                      for k, v in conv.items()},
+            "fp8_workload": {"shape": "N=32 C=512 H=W=14 K=512 F(4x3)", "paths": fp8},
+            "grouped": grouped,
+            "serving": serving,
+        },
+        "roofline": roof,
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(x, wt)
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 
