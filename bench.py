@@ -251,7 +251,7 @@ def bench_serving(torch, wc, args, tuning):
             dx, dw = dev(torch, x), dev(torch, wt)
             fid = eng.register_filter(dw.data_ptr(), k, c)
             ring = [torch.empty((nmax, k, hw, hw), dtype=torch.float16, device="cuda")
-                    for _ in range(8)]
+                    for _ in range(32)]
             cls.append({"x": x, "wt": wt, "dx": dx, "dw": dw, "fid": fid, "ring": ring, "c": c,
                         "k": k, "hw": hw})
         torch.cuda.synchronize()
@@ -259,32 +259,40 @@ def bench_serving(torch, wc, args, tuning):
         capacity = None
         # first pass: every request offered at once (saturation) measures the
         # capacity; then Poisson arrivals at 30/60/90% of it
-        for frac in (None, 0.3, 0.6, 0.9):
+        for frac in (None, 0.05, 0.3, 0.6, 0.9):
             qps = 1e12 if frac is None else frac * capacity
-            srv = wc.Server(eng, slots=4, max_batch=16, use_graph=True)
+            # the request stream is drawn up front and the checked requests'
+            # outputs allocated before the clock starts (no allocation while
+            # requests are in flight)
+            plan = [(int(rng.integers(len(CLASSES))), int(rng.integers(1, nmax + 1)),
+                     float(rng.exponential(1.0 / qps))) for _ in range(nreq)]
+            outs = {r: torch.empty((n, cls[ci]["k"], cls[ci]["hw"], cls[ci]["hw"]),
+                                   dtype=torch.float16, device="cuda")
+                    for r, (ci, n, _) in enumerate(plan) if r % 100 == 0}
+            torch.cuda.synchronize()
+            srv = wc.Server(eng, slots=args.slots, max_batch=args.max_batch, use_graph=True)
             reqs, checks = [], []
             F = 0.0
             try:
                 t = time.perf_counter()
                 t0 = t
-                for r in range(nreq):
-                    ci = int(rng.integers(len(CLASSES)))
-                    n = int(rng.integers(1, nmax + 1))
+                for r, (ci, n, gap) in enumerate(plan):
                     cl = cls[ci]
-                    sample = r % 100 == 0
-                    y = (torch.empty((n, cl["k"], cl["hw"], cl["hw"]), dtype=torch.float16,
-                                     device="cuda") if sample else cl["ring"][r % 8])
-                    if sample:
-                        torch.cuda.synchronize()
-                    while time.perf_counter() < t:
-                        pass
+                    sample = r in outs
+                    y = outs[r] if sample else cl["ring"][r % 32]
+                    while True:  # sleep to the arrival time (spin only the last 150 us)
+                        d = t - time.perf_counter()
+                        if d <= 0:
+                            break
+                        if d > 3e-4:
+                            time.sleep(d - 1.5e-4)
                     rid = srv.submit(cl["fid"], cl["dx"].data_ptr(), y.data_ptr(), n, cl["hw"],
                                      cl["hw"])
                     reqs.append(rid)
                     F += flops(n, cl["c"], cl["hw"], cl["hw"], cl["k"])
                     if sample:
                         checks.append((rid, ci, n, y))
-                    t += rng.exponential(1.0 / qps)
+                    t += gap
                 srv.drain()
                 wall = time.perf_counter() - t0
                 lat = np.array([srv.poll(rid) for rid in reqs])
@@ -304,6 +312,8 @@ def bench_serving(torch, wc, args, tuning):
                    "batches": int(st.batches), "max_batch": int(st.max_batch_problems),
                    "max_inflight_batches": int(st.max_inflight_batches),
                    "pool_peak_mib": round(st.pool_peak / 2**20, 1),
+                   "mean_batch_gpu_us": round(st.mean_batch_gpu_us, 1),
+                   "mean_queue_us": round(st.mean_queue_us, 1),
                    "checked": len(errs), "max_rel_l2": float(f"{max(errs):.3e}") if errs else None}
             if frac is None:
                 capacity = nreq / wall
@@ -346,6 +356,10 @@ def main():
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default=None, help="run one conv path only (for profiling)")
+    ap.add_argument("--serving-only", action="store_true", help="run the serving workload only")
+    ap.add_argument("--grouped-only", action="store_true", help="run the grouped workload only")
+    ap.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
+    ap.add_argument("--max-batch", type=int, default=16, help="serving: requests per batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -367,6 +381,12 @@ def main():
 
     tuning = os.path.join(ROOT, "dcu-winograd-conv2d_amd", "tuning", "mi355x.tsv")
     tuning = tuning if os.path.exists(tuning) else None
+    if args.serving_only:
+        print(json.dumps(bench_serving(torch, wc, args, tuning)), flush=True)
+        return
+    if args.grouped_only:
+        print(json.dumps(bench_grouped(torch, wc, args, tuning)), flush=True)
+        return
     conv, F = bench_conv(torch, wc, args)
     if args.only:
         for v in conv.values():

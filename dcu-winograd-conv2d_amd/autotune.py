@@ -1,7 +1,7 @@
 """Autotuner of the grouped GEMM tile shape and reduction split.
 
 For every shape class (3x3 filter K x C, image H x W) and a ladder of batch
-sizes N it runs one problem alone through the grouped launch with every tile
+sizes N it runs the problem through the grouped launch with every tile
 configuration of the GEMM table and every reduction split that divides the
 work sensibly, times each with HIP events, and writes the fastest as a line
 
@@ -9,9 +9,12 @@ work sensibly, times each with HIP events, and writes the fastest as a line
 
 of a tuning table that wino_engine_load_tuning reads (P = tiles of the
 problem; consecutive ladder points split the P axis at their midpoints).
-The tile choice never changes results: every configuration accumulates each
-output in the same order (checked by the GPU tests); the split changes the
-FP32 summation order within the stated tolerance.
+A small problem is measured as several copies in one batch (enough work for
+~20 GFLOP per launch, at most 64 copies), so that the choice reflects its
+throughput inside a loaded grouped launch rather than the fixed launch
+latency.  The tile choice never changes results: every configuration
+accumulates each output in the same order (checked by the GPU tests); the
+split changes the FP32 summation order within the stated tolerance.
 
     python autotune.py [--out tuning/mi355x.tsv] [--classes C,K,HW ...]
 """
@@ -25,7 +28,7 @@ if HERE not in sys.path:
 
 DEFAULT_CLASSES = [(64, 64, 56), (128, 128, 28), (256, 256, 14), (512, 512, 7), (256, 256, 28)]
 LADDER = [1, 2, 4, 8, 16, 32, 64]
-N_CFG = 5
+N_CFG = 6
 
 
 def main():
@@ -52,20 +55,29 @@ def main():
             rows = []
             for n in LADDER:
                 x = torch.randn((n, c, hw, hw), device="cuda").half()
-                y = torch.empty((n, k, hw, hw), device="cuda", dtype=torch.float16)
+                copies = int(min(64, max(1, -(-2e10 // (2.0 * n * k * hw * hw * c * 9)))))
+                ys = [torch.empty((n, k, hw, hw), device="cuda", dtype=torch.float16)
+                      for _ in range(copies)]
                 torch.cuda.synchronize()
-                b = wc.Batch(eng, 1)
+                b = wc.Batch(eng, copies)
                 try:
-                    b.add(fid, x.data_ptr(), y.data_ptr(), n, hw, hw)
+                    for y in ys:
+                        b.add(fid, x.data_ptr(), y.data_ptr(), n, hw, hw)
                     inf = b.info(0)
                     m, P = inf.m, inf.tiles
                     nsteps = inf.cp // (32 if args.precision == 0 else 128)
                     best = None
                     for cfg in range(N_CFG):
                         for split in (1, 2, 4):
-                            if split > nsteps:
+                            # a reduction split writes FP32 partials: only
+                            # for problems too small to fill the device alone
+                            if split > nsteps or (split > 1 and P > 256):
                                 continue
-                            b.set_tile(0, cfg, split)
+                            try:
+                                for j in range(copies):
+                                    b.set_tile(j, cfg, split)
+                            except wc.WinoError:
+                                continue  # tile not available for this K / precision
                             for _ in range(3):
                                 b.launch()
                             torch.cuda.synchronize()
@@ -76,12 +88,12 @@ def main():
                                 b.launch()
                             e1.record()
                             torch.cuda.synchronize()
-                            us = e0.elapsed_time(e1) * 1e3 / args.reps
+                            us = e0.elapsed_time(e1) * 1e3 / args.reps / copies
                             if best is None or us < best[0]:
                                 best = (us, cfg, split)
                     rows.append((m, P, best))
                     print(f"C={c} K={k} HW={hw} N={n} m={m} P={P}: cfg={best[1]} split={best[2]} "
-                          f"{best[0]:.1f} us", flush=True)
+                          f"{best[0]:.1f} us per problem ({copies} per launch)", flush=True)
                 finally:
                     b.destroy()
             # P ranges per m: midpoints between consecutive ladder points

@@ -46,20 +46,31 @@ int validate(const problem_t* pr) {
 
 // GEMM tile choice: least padded work, then enough blocks to fill 256 CUs,
 // then the larger tile (more operand reuse).
-int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision) {
+// Tile configuration by a roofline model of the batched GEMM: HBM bytes
+// (V is re-read once per row block of K, U once per XCD, M written once)
+// against MFMA time derated by the tile's operand reuse, divided by how well
+// the tiles fill 256 CUs.  kmax > 0 limits the padded K rows a tile may
+// touch (the row count of an existing U layout).
+int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax) {
+  const double b_op = precision == WINO_PREC_FP16 ? 2.0 : (precision == WINO_PREC_FP8 ? 1.0 : 2.0);
+  const double mfma = precision == WINO_PREC_FP16 ? 2.5e15 : (precision == WINO_PREC_FP8 ? 5.0e15 : 1.7e15);
   int best = 0;
-  double best_score = -1.0;
+  double best_t = 1e300;
   for (int i = 0; i < gemm_cfg_count(); ++i) {
+    if (!gemm_cfg_ok(i, precision)) continue;
     const GemmCfg c = gemm_cfg(i);
     if (precision != WINO_PREC_FP16 && c.bk * c.bp < 64 * 64) continue;
+    if (kmax && round_up(K, c.bk) > kmax) continue;
     const double kp = (double)round_up(K, c.bk), pp = (double)round_up(P, c.bp);
-    const double eff = (double)K * (double)P / (kp * pp);
-    const double blocks = kp / c.bk * pp / c.bp * t2;
+    const double kb = kp / c.bk, pb = pp / c.bp;
+    const double bytes = t2 * (kb * pp * b_op + (pb < 8 ? pb : 8) * kp * b_op + kp * pp * 2.0);
+    const double reuse = (double)c.bk * c.bp / (c.bk + c.bp) / 64.0;  // 1 for 128 x 128
+    const double tm = 2.0 * t2 * kp * pp / (mfma * 0.5 * (reuse < 1.0 ? reuse : 1.0));
+    const double blocks = kb * pb * t2;
     const double fill = blocks >= 512.0 ? 1.0 : blocks / 512.0;
-    const double reuse = (double)c.bk * c.bp / (c.bk + c.bp) / 64.0;  // 1.0 for 128x128
-    const double score = eff * fill * (0.75 + 0.25 * reuse);
-    if (score > best_score) {
-      best_score = score;
+    const double t = (bytes / 6.0e12 > tm ? bytes / 6.0e12 : tm) / fill;
+    if (t < best_t) {
+      best_t = t;
       best = i;
     }
   }
@@ -98,7 +109,7 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     pl->ws_bytes = align256(pl->u_bytes) + 256;
     return WINO_OK;
   }
-  pl->gemm_cfg = choose_gemm_cfg(pr->k, pl->P, pl->t2, precision);
+  pl->gemm_cfg = choose_gemm_cfg(pr->k, pl->P, pl->t2, precision, 0);
   const GemmCfg gc = gemm_cfg(pl->gemm_cfg);
   pl->bk = gc.bk;
   pl->bp = gc.bp;

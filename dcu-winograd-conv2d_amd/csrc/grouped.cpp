@@ -155,19 +155,24 @@ struct BPlan {
   uint64_t items;     // GEMM work items (tiles x split)
 };
 
-// Work items one batch can hold (24 bytes each, staged in pinned memory).
-constexpr uint64_t kBatchMaxItems = (uint64_t)1 << 18;
+// GEMM work items one batch can hold (the kernels index them in 32 bits).
+constexpr uint64_t kBatchMaxItems = (uint64_t)1 << 31;
+
+// A tile may only touch the K rows of the filter's U layout.
+static bool tile_fits(int cfg, int precision, uint32_t K) {
+  return gemm_cfg_ok(cfg, precision) && ru(K, gemm_cfg(cfg).bk) <= ru(K, 128);
+}
 
 static void lookup_tile(const wino_engine* e, uint32_t m, uint32_t K, uint32_t C, uint64_t P,
                         int t2, int* cfg, int* split, uint32_t nsteps) {
   for (const TuneEntry& te : e->tune)
     if (te.m == m && te.K == K && te.C == C && P >= te.pmin && P <= te.pmax &&
-        te.cfg >= 0 && te.cfg < gemm_cfg_count()) {
+        tile_fits(te.cfg, e->precision, K)) {
       *cfg = te.cfg;
       *split = std::max(1, std::min<int>(te.split, (int)nsteps));
       return;
     }
-  *cfg = choose_gemm_cfg(K, P, t2, e->precision);
+  *cfg = choose_gemm_cfg(K, P, t2, e->precision, (uint32_t)ru(K, 128));
   const GemmCfg gc = gemm_cfg(*cfg);
   const uint64_t tiles = (uint64_t)((K + gc.bk - 1) / gc.bk) * ((P + gc.bp - 1) / gc.bp) * t2;
   // a problem too small to fill the device on its own splits its reduction
@@ -196,7 +201,7 @@ static int plan_batched(const wino_engine* e, uint32_t K, uint32_t C, uint32_t n
   pl.Kp = (uint32_t)ru(K, 128);
   const uint32_t nsteps = e->precision == WINO_PREC_FP16 ? pl.Cp / 32 : pl.Cp / 128;
   if (cfg < 0) lookup_tile(e, m, K, C, pl.P, pl.t2, &cfg, &split, nsteps);
-  if (cfg >= gemm_cfg_count() || split < 1) return WINO_ERR_INVALID;
+  if (!tile_fits(cfg, e->precision, K) || split < 1) return WINO_ERR_INVALID;
   split = std::min<int>(split, (int)nsteps);
   const GemmCfg gc = gemm_cfg(cfg);
   pl.gemm_cfg = cfg;
@@ -262,10 +267,14 @@ struct wino_batch {
     GProb gp;
   };
   std::vector<Entry> entries;
-  // device descriptors: header | probs | items (one contiguous allocation)
+  // device descriptors: header | probs (one allocation), rebuilt and copied
+  // only when the batch changed or the launch stream changed
   char* dev = nullptr;
   char* host = nullptr;  // pinned staging of the same layout
-  size_t probs_off = 0, items_off = 0, bytes = 0;
+  size_t probs_off = 0, bytes = 0;
+  bool dirty = true;
+  hipStream_t last_stream = nullptr;
+  hipEvent_t copied = nullptr;  // the last descriptor copy has read `host`
   uint32_t grid_in = 0, grid_gemm = 0, grid_out = 0, grid_amax = 0;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -276,12 +285,11 @@ namespace wino {
 static int batch_enqueue_kernels(wino_batch* b, hipStream_t st) {
   const GroupHeader* hdr = (const GroupHeader*)b->dev;
   const GProb* probs = (const GProb*)(b->dev + b->probs_off);
-  const GItem* items = (const GItem*)(b->dev + b->items_off);
   const int prec = b->eng->precision;
   hipError_t e = hipSuccess;
   if (prec != WINO_PREC_FP16) e = launch_amax_grouped(hdr, probs, b->grid_amax, st);
   if (e == hipSuccess) e = launch_input_grouped(prec, hdr, probs, b->grid_in, st);
-  if (e == hipSuccess) e = launch_gemm_grouped(prec, hdr, probs, items, b->grid_gemm, st);
+  if (e == hipSuccess) e = launch_gemm_grouped(prec, hdr, probs, b->grid_gemm, st);
   if (e == hipSuccess) e = launch_output_grouped(hdr, probs, b->grid_out, st);
   return hipst(e);
 }
@@ -449,12 +457,13 @@ int wino_batch_create(wino_engine* e, int max_problems, wino_batch** out) {
   b->max_probs = max_problems;
   b->max_items = kBatchMaxItems;
   b->probs_off = 256;
-  b->items_off = a256(b->probs_off + sizeof(GProb) * (size_t)max_problems);
-  b->bytes = b->items_off + sizeof(GItem) * b->max_items;
+  b->bytes = a256(b->probs_off + sizeof(GProb) * (size_t)max_problems);
   if (hipMalloc((void**)&b->dev, b->bytes) != hipSuccess ||
-      hipHostMalloc((void**)&b->host, b->bytes, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&b->host, b->bytes, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&b->copied, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
     if (b->dev) (void)hipFree(b->dev);
+    if (b->host) (void)hipHostFree(b->host);
     delete b;
     return WINO_ERR_ALLOC;
   }
@@ -478,6 +487,7 @@ int wino_batch_reset(wino_batch* b) {
   for (auto& en : b->entries)
     if (en.has_ws) b->eng->pool.release(en.pool_off, en.bp.footprint);
   b->entries.clear();
+  b->dirty = true;
   return WINO_OK;
 }
 
@@ -489,6 +499,7 @@ int wino_batch_destroy(wino_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->dev) (void)hipFree(b->dev);
   if (b->host) (void)hipHostFree(b->host);
+  if (b->copied) (void)hipEventDestroy(b->copied);
   delete b;
   return WINO_OK;
 }
@@ -560,6 +571,7 @@ int wino_batch_add(wino_batch* b, int filter_id, const void* in, void* out, uint
   en.has_ws = true;
   fill_gprob(b, en, in, out, n, h, w);
   b->entries.push_back(en);
+  b->dirty = true;
   if (index) *index = (int)b->entries.size() - 1;
   return WINO_OK;
 }
@@ -583,6 +595,7 @@ int wino_batch_set_tile(wino_batch* b, int index, int cfg, int split) {
   en.bp = bp;
   en.pool_off = off;
   fill_gprob(b, en, old.x, old.y, n, h, w);
+  b->dirty = true;
   return WINO_OK;
 }
 
@@ -618,63 +631,63 @@ int wino_batch_info(wino_batch* b, int index, wino_info_t* info) {
   return WINO_OK;
 }
 
-// Writes header, problem descriptors and LPT-ordered work items into the
-// pinned staging buffer; returns its used size.
+// Writes the header and the problem descriptors into the pinned staging
+// buffer, problems in decreasing cost per GEMM item (LPT order for the
+// persistent GEMM's round-robin); returns the used size.
 static size_t build_descriptors(wino_batch* b) {
   GroupHeader* hdr = (GroupHeader*)b->host;
   GProb* probs = (GProb*)(b->host + b->probs_off);
-  GItem* items = (GItem*)(b->host + b->items_off);
   memset(hdr, 0, sizeof(*hdr));
-  uint32_t in_first = 0, out_first = 0, amax_first = 0;
-  struct Work {
-    uint64_t cost;
-    GItem it;
-  };
-  std::vector<Work> work;
-  for (size_t i = 0; i < b->entries.size(); ++i) {
-    GProb g = b->entries[i].gp;
+  std::vector<uint32_t> order(b->entries.size());
+  std::vector<uint64_t> cost(b->entries.size());
+  for (size_t i = 0; i < order.size(); ++i) {
     const BPlan& bp = b->entries[i].bp;
+    order[i] = (uint32_t)i;
+    cost[i] = (uint64_t)bp.pl.bk * bp.pl.bp * (b->entries[i].gp.nsteps / bp.split);
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
+  uint32_t in_first = 0, out_first = 0, amax_first = 0;
+  uint64_t item_first = 0;
+  for (size_t j = 0; j < order.size(); ++j) {
+    const auto& en = b->entries[order[j]];
+    GProb g = en.gp;
     g.in_first = in_first;
     in_first += g.in_count;
     g.out_first = out_first;
     out_first += g.out_count;
     g.amax_first = amax_first;
     amax_first += g.amax_count;
-    probs[i] = g;
-    const uint64_t pblocks = bp.pl.Pp / bp.pl.bp;
-    const uint64_t cost = (uint64_t)bp.pl.bk * bp.pl.bp * (g.nsteps / bp.split);
-    for (uint32_t t = 0; t < g.t2; ++t)
-      for (uint32_t kb = 0; kb < bp.kblocks; ++kb)
-        for (uint64_t pb = 0; pb < pblocks; ++pb)
-          for (uint32_t sp = 0; sp < bp.split; ++sp) {
-            Work wk;
-            wk.cost = cost;
-            wk.it.prob = (uint32_t)i;
-            wk.it.t = t;
-            wk.it.kb = kb;
-            wk.it.sp = sp;
-            wk.it.pb = pb;
-            work.push_back(wk);
-          }
+    g.kblocks = en.bp.kblocks;
+    g.pblocks = (uint32_t)(en.bp.pl.Pp / en.bp.pl.bp);
+    g.item_first = (uint32_t)item_first;
+    g.item_count = (uint32_t)en.bp.items;
+    item_first += en.bp.items;
+    probs[j] = g;
   }
-  std::stable_sort(work.begin(), work.end(),
-                   [](const Work& a, const Work& c) { return a.cost > c.cost; });
-  const size_t nitems = work.size();  // <= max_items: checked as problems are added
-  for (size_t i = 0; i < nitems; ++i) items[i] = work[i].it;
   hdr->nprob = (uint32_t)b->entries.size();
   hdr->in_blocks = in_first;
   hdr->out_blocks = out_first;
-  hdr->nitems = (uint32_t)nitems;
+  hdr->nitems = (uint32_t)item_first;  // < 2^31: checked as problems are added
   hdr->amax_blocks = amax_first;
-  return b->items_off + nitems * sizeof(GItem);
+  return b->probs_off + order.size() * sizeof(GProb);
 }
 
 int wino_batch_launch(wino_batch* b, void* stream) {
   if (!b) return WINO_ERR_INVALID;
   hipStream_t st = (hipStream_t)stream;
-  const size_t used = build_descriptors(b);
-  int rc = hipst(hipMemcpyAsync(b->dev, b->host, used, hipMemcpyHostToDevice, st));
-  if (rc) return rc;
+  if (b->dirty || st != b->last_stream) {
+    // the previous copy out of the staging buffer must be done before it
+    // is rewritten
+    int rc = hipst(hipEventSynchronize(b->copied));
+    if (rc) return rc;
+    const size_t used = build_descriptors(b);
+    rc = hipst(hipMemcpyAsync(b->dev, b->host, used, hipMemcpyHostToDevice, st));
+    if (!rc) rc = hipst(hipEventRecord(b->copied, st));
+    if (rc) return rc;
+    b->dirty = false;
+    b->last_stream = st;
+  }
   if (b->exec) return hipst(hipGraphLaunch(b->exec, st));
   return batch_enqueue_kernels(b, st);
 }
@@ -727,6 +740,7 @@ struct wino_server {
     wino_batch* batch = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t event = nullptr;
+    hipEvent_t start = nullptr;  // batch start, for the device time per batch
     bool busy = false;
     bool captured = false;
     std::vector<uint64_t> reqs;
@@ -740,6 +754,8 @@ struct wino_server {
   std::atomic<bool> stop{false};
   uint64_t next_id = 1;
   wino_server_stats_t stats;
+  double gpu_us_sum = 0, queue_us_sum = 0;
+  uint64_t launched = 0;  // requests launched in a batch
   int error = WINO_OK;
 };
 
@@ -757,8 +773,14 @@ static void server_loop(wino_server* s) {
         continue;
       }
       const double t = now_us();
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, sl.start, sl.event) != hipSuccess) {
+        (void)hipGetLastError();
+        ms = 0.0f;
+      }
       {
         std::lock_guard<std::mutex> lk(s->mu);
+        s->gpu_us_sum += 1e3 * ms;
         for (uint64_t id : sl.reqs) {
           auto it = s->reqs.find(id);
           if (it != s->reqs.end()) {
@@ -806,6 +828,7 @@ static void server_loop(wino_server* s) {
         st = wino_batch_capture(sl.batch, sl.stream);
         sl.captured = st == WINO_OK;
       }
+      if (!st) st = hipst(hipEventRecord(sl.start, sl.stream));
       if (!st) st = wino_batch_launch(sl.batch, sl.stream);
       if (!st) st = hipst(hipEventRecord(sl.event, sl.stream));
       if (st) s->error = st;
@@ -813,7 +836,10 @@ static void server_loop(wino_server* s) {
       sl.busy = true;
       ++inflight;
       progressed = true;
+      const double tl = now_us();
       std::lock_guard<std::mutex> lk(s->mu);
+      for (uint64_t id : taken) s->queue_us_sum += tl - s->reqs[id].t_submit;
+      s->launched += taken.size();
       ++s->stats.batches;
       s->stats.max_batch_problems = std::max<uint32_t>(s->stats.max_batch_problems, (uint32_t)taken.size());
       s->stats.max_inflight_batches = std::max<uint32_t>(s->stats.max_inflight_batches, inflight);
@@ -849,12 +875,24 @@ int wino_server_create(wino_engine* e, int slots, int max_batch, int use_graph,
   for (auto& sl : s->slots) {
     int st = wino_batch_create(e, max_batch, &sl.batch);
     if (!st) st = hipst(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-    if (!st) st = hipst(hipEventCreateWithFlags(&sl.event, hipEventDisableTiming));
+    if (!st) st = hipst(hipEventCreate(&sl.event));
+    if (!st) st = hipst(hipEventCreate(&sl.start));
+    // capture each slot's graph now (the batch is empty; the kernels read
+    // the descriptors at replay), not on the first request
+    if (!st && s->use_graph) {
+      st = wino_batch_capture(sl.batch, sl.stream);
+      sl.captured = st == WINO_OK;
+      // one replay of the empty batch moves the graph's first-launch setup
+      // out of the first request's latency
+      if (!st) st = wino_batch_launch(sl.batch, sl.stream);
+      if (!st) st = hipst(hipStreamSynchronize(sl.stream));
+    }
     if (st) {
       for (auto& q : s->slots) {
         if (q.batch) wino_batch_destroy(q.batch);
         if (q.stream) (void)hipStreamDestroy(q.stream);
         if (q.event) (void)hipEventDestroy(q.event);
+        if (q.start) (void)hipEventDestroy(q.start);
       }
       delete s;
       return st;
@@ -921,6 +959,8 @@ int wino_server_stats(wino_server* s, wino_server_stats_t* st) {
   std::lock_guard<std::mutex> lk(s->mu);
   *st = s->stats;
   st->pool_peak = s->eng->pool.peak();
+  st->mean_batch_gpu_us = s->stats.batches ? s->gpu_us_sum / (double)s->stats.batches : 0.0;
+  st->mean_queue_us = s->launched ? s->queue_us_sum / (double)s->launched : 0.0;
   return WINO_OK;
 }
 
@@ -934,6 +974,7 @@ int wino_server_destroy(wino_server* s) {
     if (sl.batch) wino_batch_destroy(sl.batch);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.event) (void)hipEventDestroy(sl.event);
+    if (sl.start) (void)hipEventDestroy(sl.start);
   }
   const int err = s->error;
   delete s;

@@ -286,6 +286,22 @@ __device__ inline void gemm_tile_f8(char* smem, const intx8* __restrict__ U,
 }
 
 // ------------------------------------------------------- single problem
+// Block -> tile order for the single-problem kernels.  Workgroups are
+// dispatched round-robin over the 8 XCDs (each with its own L2); renumbering
+// so that XCD j runs the j-th contiguous range of tiles (tile position t
+// slowest) keeps each U[t] in one L2 instead of all eight.
+__device__ inline void tile_of_block(uint32_t& pb, uint32_t& kb, uint32_t& t) {
+  const uint32_t gx = gridDim.x, gy = gridDim.y;
+  const uint32_t G = gx * gy * gridDim.z;
+  const uint32_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const uint32_t per = G / 8u;
+  const uint32_t L = b < per * 8u ? (b % 8u) * per + b / 8u : b;
+  pb = L % gx;
+  const uint32_t r = L / gx;
+  kb = r % gy;
+  t = r / gy;
+}
+
 template <int WK, int WP, int TK, int TP>
 __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restrict__ U,
                                                           const half8* __restrict__ V,
@@ -293,8 +309,10 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
                                                           uint64_t Pp, uint32_t C8p,
                                                           uint32_t T2) {
   __shared__ __attribute__((aligned(16))) char smem[TileShape<WK, WP, TK, TP>::lds_f16()];
-  gemm_tile_f16<WK, WP, TK, TP>(smem, U, V, M, nullptr, Kp, Pp, C8p, T2, blockIdx.z,
-                                blockIdx.y, blockIdx.x, 0, (int)(C8p / 4), 0);
+  uint32_t pb, kb, t;
+  tile_of_block(pb, kb, t);
+  gemm_tile_f16<WK, WP, TK, TP>(smem, U, V, M, nullptr, Kp, Pp, C8p, T2, t, kb, pb, 0,
+                                (int)(C8p / 4), 0);
 }
 
 template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
@@ -303,9 +321,10 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
     const float* __restrict__ us, const float* __restrict__ vs, uint32_t Kp, uint64_t Pp,
     uint32_t C32p, uint32_t T2) {
   extern __shared__ __attribute__((aligned(16))) char dsmem[];
+  uint32_t pb, kb, t;
+  tile_of_block(pb, kb, t);
   gemm_tile_f8<WK, WP, TK, TP, PLANES, LOLO>(dsmem, U, V, M, nullptr, us, vs, Kp, Pp, C32p, T2,
-                                             blockIdx.z, blockIdx.y, blockIdx.x, 0,
-                                             (int)(C32p / 4), 0);
+                                             t, kb, pb, 0, (int)(C32p / 4), 0);
 }
 
 // ------------------------------------------------------------ tile table
@@ -317,8 +336,12 @@ static const int kCfg[][4] = {
     {2, 2, 2, 2},  // 2:  64 x  64
     {1, 4, 2, 2},  // 3:  32 x 128
     {1, 4, 1, 1},  // 4:  16 x  64
+    {4, 1, 4, 4},  // 5: 256 x  64 (fp16 only: all of K <= 256 in one tile, V read once)
 };
 int gemm_cfg_count() { return (int)(sizeof(kCfg) / sizeof(kCfg[0])); }
+bool gemm_cfg_ok(int idx, int precision) {
+  return idx >= 0 && idx < gemm_cfg_count() && (precision == WINO_PREC_FP16 || idx < 5);
+}
 GemmCfg gemm_cfg(int idx) {
   GemmCfg c;
   c.bk = kCfg[idx][0] * kCfg[idx][2] * 16;
@@ -364,6 +387,7 @@ hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M
       case 1: run_f16<2, 2, 2, 4>(pl, U, V, M, st); break;
       case 2: run_f16<2, 2, 2, 2>(pl, U, V, M, st); break;
       case 3: run_f16<1, 4, 2, 2>(pl, U, V, M, st); break;
+      case 5: run_f16<4, 1, 4, 4>(pl, U, V, M, st); break;
       default: run_f16<1, 4, 1, 1>(pl, U, V, M, st); break;
     }
   } else {
@@ -379,19 +403,42 @@ hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M
 }
 
 // --------------------------------------------------------- grouped GEMM
-// Persistent kernel over the work items of a batch (kernels_grouped.hip
-// builds the descriptors).  Each item is one tile of one problem; the
-// problem's tile shape and reduction split come from its descriptor.
+// Persistent kernel over the work items of a batch.  Problems are laid out
+// in the descriptor array in decreasing cost per item (the host orders
+// them), so the round-robin over items hands out the longest tiles first.
+// An item's problem is tracked by a forward cursor and the item is decoded
+// arithmetically; the problem's tile shape and reduction split
+// come from its descriptor.
+struct GWork {
+  uint32_t t, kb, sp, s0, s1;
+  uint64_t pb;
+};
+
+__device__ inline GWork decode_item(const GProb& pr, uint32_t i) {
+  GWork w;
+  uint32_t r = i - pr.item_first;
+  w.sp = r % pr.split;
+  r /= pr.split;
+  w.pb = r % pr.pblocks;
+  r /= pr.pblocks;
+  w.kb = r % pr.kblocks;
+  w.t = r / pr.kblocks;
+  w.s0 = w.sp * pr.nsteps / pr.split;
+  w.s1 = (w.sp + 1) * pr.nsteps / pr.split;
+  return w;
+}
+
 __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __restrict__ hdr,
-                                                          const GProb* __restrict__ probs,
-                                                          const GItem* __restrict__ items) {
+                                                          const GProb* __restrict__ probs) {
   extern __shared__ __attribute__((aligned(16))) char gsmem[];
-  const uint32_t n = hdr->nitems;
+  const uint32_t n = hdr->nitems, np = hdr->nprob;
+  ProbCursor<&GProb::item_first> cur;
+  cur.init(probs, np);
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const GItem it = items[i];
-    const GProb& pr = probs[it.prob];
-    const int s0 = (int)(it.sp * pr.nsteps / pr.split);
-    const int s1 = (int)((it.sp + 1) * pr.nsteps / pr.split);
+    cur.seek(probs, np, i);
+    const GProb& pr = probs[cur.p];
+    const GWork it = decode_item(pr, i);
+    const int s0 = (int)it.s0, s1 = (int)it.s1;
     float* m32 = pr.split > 1 ? pr.M32 : nullptr;
     const half8* U = (const half8*)pr.U;
     const half8* V = (const half8*)pr.V;
@@ -400,6 +447,7 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __r
       case 1: gemm_tile_f16<2, 2, 2, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
       case 2: gemm_tile_f16<2, 2, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
       case 3: gemm_tile_f16<1, 4, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+      case 5: gemm_tile_f16<4, 1, 4, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
       default: gemm_tile_f16<1, 4, 1, 1>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
     }
   }
@@ -407,15 +455,16 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __r
 
 template <int PLANES>
 __global__ __launch_bounds__(256) void k_gemm_grouped_f8(const GroupHeader* __restrict__ hdr,
-                                                         const GProb* __restrict__ probs,
-                                                         const GItem* __restrict__ items) {
+                                                         const GProb* __restrict__ probs) {
   extern __shared__ __attribute__((aligned(16))) char gsmem[];
-  const uint32_t n = hdr->nitems;
+  const uint32_t n = hdr->nitems, np = hdr->nprob;
+  ProbCursor<&GProb::item_first> cur;
+  cur.init(probs, np);
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const GItem it = items[i];
-    const GProb& pr = probs[it.prob];
-    const int s0 = (int)(it.sp * pr.nsteps / pr.split);
-    const int s1 = (int)((it.sp + 1) * pr.nsteps / pr.split);
+    cur.seek(probs, np, i);
+    const GProb& pr = probs[cur.p];
+    const GWork it = decode_item(pr, i);
+    const int s0 = (int)it.s0, s1 = (int)it.s1;
     float* m32 = pr.split > 1 ? pr.M32 : nullptr;
     const intx8* U = (const intx8*)pr.U;
     const intx8* V = (const intx8*)pr.V;
@@ -443,19 +492,22 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f8(const GroupHeader* __re
 }
 
 size_t grouped_gemm_lds(int precision) {
-  if (precision == WINO_PREC_FP16) return (size_t)TileShape<2, 2, 4, 4>::lds_f16();
+  if (precision == WINO_PREC_FP16) {
+    const int a = TileShape<2, 2, 4, 4>::lds_f16(), b = TileShape<4, 1, 4, 4>::lds_f16();
+    return (size_t)(a > b ? a : b);
+  }
   return (size_t)TileShape<2, 2, 4, 4>::lds_f8(precision == WINO_PREC_FP8X2 ? 2 : 1);
 }
 
 hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
-                               const GItem* items, uint32_t grid, hipStream_t st) {
+                               uint32_t grid, hipStream_t st) {
   const size_t lds = grouped_gemm_lds(precision);
   if (precision == WINO_PREC_FP16) {
-    hipLaunchKernelGGL(k_gemm_grouped_f16, dim3(grid), dim3(256), lds, st, hdr, probs, items);
+    hipLaunchKernelGGL(k_gemm_grouped_f16, dim3(grid), dim3(256), lds, st, hdr, probs);
   } else if (precision == WINO_PREC_FP8) {
-    hipLaunchKernelGGL(k_gemm_grouped_f8<1>, dim3(grid), dim3(256), lds, st, hdr, probs, items);
+    hipLaunchKernelGGL(k_gemm_grouped_f8<1>, dim3(grid), dim3(256), lds, st, hdr, probs);
   } else {
-    hipLaunchKernelGGL(k_gemm_grouped_f8<2>, dim3(grid), dim3(256), lds, st, hdr, probs, items);
+    hipLaunchKernelGGL(k_gemm_grouped_f8<2>, dim3(grid), dim3(256), lds, st, hdr, probs);
   }
   return hipGetLastError();
 }

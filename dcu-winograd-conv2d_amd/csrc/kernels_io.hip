@@ -385,8 +385,8 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 }
 
 // ------------------------------------------------------------- grouped
-// Fixed grids; virtual block v is mapped to its problem by a scan over the
-// problems' first-block offsets (batches hold at most a few hundred problems).
+// Fixed grids; virtual block v is mapped to its problem by a forward cursor
+// over the problems' first-block offsets.
 
 template <int PREC>
 __global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __restrict__ hdr,
@@ -394,10 +394,11 @@ __global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __rest
   extern __shared__ __attribute__((aligned(16))) half_t gxs[];
   constexpr int CG = PREC == 0 ? 8 : 32;
   const uint32_t total = hdr->in_blocks, np = hdr->nprob;
+  ProbCursor<&GProb::in_first> cur;
+  cur.init(probs, np);
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
-    uint32_t p = 0;
-    while (p + 1 < np && probs[p + 1].in_first <= v) ++p;
-    const GProb& pr = probs[p];
+    cur.seek(probs, np, v);
+    const GProb& pr = probs[cur.p];
     const uint32_t local = v - pr.in_first;
     const uint32_t bx = local % pr.in_gx, cg = local / pr.in_gx;
     const bool evenw = (pr.ig.W & 1u) == 0;
@@ -419,10 +420,11 @@ __global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __rest
 __global__ __launch_bounds__(256) void k_output_grouped(const GroupHeader* __restrict__ hdr,
                                                         const GProb* __restrict__ probs) {
   const uint32_t total = hdr->out_blocks, np = hdr->nprob;
+  ProbCursor<&GProb::out_first> cur;
+  cur.init(probs, np);
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
-    uint32_t p = 0;
-    while (p + 1 < np && probs[p + 1].out_first <= v) ++p;
-    const GProb& pr = probs[p];
+    cur.seek(probs, np, v);
+    const GProb& pr = probs[cur.p];
     const uint64_t idx = (uint64_t)(v - pr.out_first) * 256u + threadIdx.x;
     if (pr.m == 2)
       output_pair<2>(pr.M, pr.M32, (int)pr.split, pr.y, pr.K, pr.ig.TH, pr.ig.TW, pr.P, pr.Pp,

@@ -179,10 +179,11 @@ __global__ __launch_bounds__(256) void k_amax_grouped(const GroupHeader* __restr
                                                       const GProb* __restrict__ probs) {
   __shared__ float red[4];
   const uint32_t total = hdr->amax_blocks, np = hdr->nprob;
+  ProbCursor<&GProb::amax_first> cur;
+  cur.init(probs, np);
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
-    uint32_t p = 0;
-    while (p + 1 < np && probs[p + 1].amax_first <= v) ++p;
-    const GProb& pr = probs[p];
+    cur.seek(probs, np, v);
+    const GProb& pr = probs[cur.p];
     const uint32_t local = v - pr.amax_first;
     const uint64_t n = (uint64_t)pr.ig.N * pr.ig.C * pr.ig.H * pr.ig.W;
     const uint64_t stride = (uint64_t)pr.amax_count * 256u;

@@ -59,7 +59,7 @@ struct Param {
 // ------------------------------------------------------------ planning
 int reference_select(const problem_t* pr);
 int validate(const problem_t* pr);
-int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision);
+int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax);
 
 // ------------------------------------------------------------ GEMM table
 struct GemmCfg {
@@ -67,6 +67,7 @@ struct GemmCfg {
   int threads;
 };
 int gemm_cfg_count();
+bool gemm_cfg_ok(int idx, int precision);  // configuration available for the precision
 GemmCfg gemm_cfg(int idx);
 
 // Input-transform window geometry (kernels_transform.hip).
@@ -92,7 +93,7 @@ struct GroupHeader {
   uint32_t nprob;
   uint32_t in_blocks;    // total virtual blocks of the input transform
   uint32_t out_blocks;   // total virtual blocks of the output transform
-  uint32_t nitems;       // GEMM work items
+  uint32_t nitems;       // GEMM work items (tiles x reduction splits, all problems)
   uint32_t amax_blocks;  // total virtual blocks of the fp8 input reduction
   uint32_t pad[3];
 };
@@ -114,15 +115,34 @@ struct GProb {
   uint32_t in_first, in_count;
   uint32_t out_first, out_count;
   uint32_t amax_first, amax_count;
-};
-struct GItem {
-  uint32_t prob, t, kb, sp;
-  uint64_t pb;
+  // GEMM work items of this problem: [item_first, item_first + item_count),
+  // item = ((t * kblocks + kb) * pblocks + pb) * split + sp
+  uint32_t item_first, item_count;
+  uint32_t kblocks, pblocks;
 };
 constexpr uint32_t kGroupInputLds = 48 * 1024;  // dynamic LDS of the grouped input kernel
 
+// Problem of virtual block / item `v` for a block that walks v upward
+// (v = blockIdx.x, blockIdx.x + gridDim.x, ...): p only moves forward, and
+// the next problem's first index is kept, so most steps load nothing.
+// FIRST selects the per-problem first-index field.
+template <uint32_t GProb::*FIRST>
+struct ProbCursor {
+  uint32_t p = 0, next = 0xffffffffu;
+  __device__ inline void init(const GProb* probs, uint32_t np) {
+    p = 0;
+    next = np > 1 ? probs[1].*FIRST : 0xffffffffu;
+  }
+  __device__ inline void seek(const GProb* probs, uint32_t np, uint32_t v) {
+    while (v >= next) {
+      ++p;
+      next = p + 1 < np ? probs[p + 1].*FIRST : 0xffffffffu;
+    }
+  }
+};
+
 hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
-                               const GItem* items, uint32_t grid, hipStream_t st);
+                               uint32_t grid, hipStream_t st);
 hipError_t grouped_gemm_prepare(int precision);
 size_t grouped_gemm_lds(int precision);
 hipError_t launch_input_grouped(int precision, const GroupHeader* hdr, const GProb* probs,

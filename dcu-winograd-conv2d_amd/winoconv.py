@@ -59,7 +59,8 @@ class ServerStats(C.Structure):
     """wino_server_stats_t, include/winoconv.h."""
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("batches", C.c_uint64), ("max_inflight_batches", C.c_uint32),
-                ("max_batch_problems", C.c_uint32), ("pool_peak", C.c_size_t)]
+                ("max_batch_problems", C.c_uint32), ("pool_peak", C.c_size_t),
+                ("mean_batch_gpu_us", C.c_double), ("mean_queue_us", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol include/*.h declares

@@ -189,6 +189,8 @@ typedef struct {
   uint32_t max_inflight_batches;  /* most batches on the device at once */
   uint32_t max_batch_problems;    /* largest batch launched               */
   size_t pool_peak;               /* most pool bytes in use at once       */
+  double mean_batch_gpu_us;       /* device time per batch (HIP events)   */
+  double mean_queue_us;           /* submit -> launch, per request        */
 } wino_server_stats_t;
 int wino_server_stats(wino_server* server, wino_server_stats_t* stats);
 

@@ -26,9 +26,9 @@ FP16, FP8, FP8X2 = 0, 1, 2
 TOL = {(2, FP16): 2e-3, (4, FP16): 1e-2, (2, FP8X2): 5e-3, (4, FP8X2): 1.5e-2}
 
 # filters (k, c, algo); problems (filter index, n, h, w)
-FILTERS = [(64, 32, AUTO), (40, 17, F43), (27, 3, F23), (96, 130, AUTO)]
+FILTERS = [(64, 32, AUTO), (40, 17, F43), (27, 3, F23), (96, 130, AUTO), (256, 40, AUTO)]
 PROBLEMS = [(0, 1, 56, 56), (0, 2, 7, 9), (1, 3, 14, 14), (1, 1, 5, 5), (2, 2, 13, 11),
-            (2, 1, 1, 1), (3, 2, 10, 10), (3, 4, 28, 28), (0, 8, 16, 16)]
+            (2, 1, 1, 1), (3, 2, 10, 10), (3, 4, 28, 28), (0, 8, 16, 16), (4, 2, 12, 10)]
 
 
 def say(*a):
@@ -105,20 +105,27 @@ def test_batch_matches_single_problem_and_fp64(wc, gpu, oracle):
         S.close()
 
 
-def test_tile_shapes_bitwise_equal(wc, gpu):
+@pytest.mark.parametrize("j", [7, 9])
+def test_tile_shapes_bitwise_equal(wc, gpu, j):
+    """Every tile shape the filter's K padding allows; 256-row tiles only
+    for K > 128 (problem 9, K = 256)."""
     S = Setup(wc, gpu, FP16)
     b = wc.Batch(S.eng, 8)
     try:
-        j = 7  # (96 x 130) filter, 4 x 28 x 28
         outs = []
-        for cfg in range(5):
+        for cfg in range(6):
             b.reset()
             y = S.fresh_out(j)
             idx = S.add(b, j, y)
-            b.set_tile(idx, cfg, 1)
-            say(f"tile cfg={cfg}")
+            try:
+                b.set_tile(idx, cfg, 1)
+            except wc.WinoError as e:
+                assert e.status == wc.ERR_INVALID and cfg == 5 and FILTERS[PROBLEMS[j][0]][0] <= 128
+                continue
+            say(f"tile cfg={cfg} problem {j}")
             b.launch()
             outs.append(gpu.to_host(y).view(np.uint16))
+        assert len(outs) == (6 if FILTERS[PROBLEMS[j][0]][0] > 128 else 5)
         for o in outs[1:]:
             assert np.array_equal(o, outs[0])
     finally:
