@@ -388,9 +388,15 @@ def main():
         print(json.dumps(bench_grouped(torch, wc, args, tuning)), flush=True)
         return
     conv, F = bench_conv(torch, wc, args)
-    if args.only:
-        for v in conv.values():
+    if args.only:  # one path and its stages alone (the profiled command)
+        out = {}
+        for name, v in conv.items():
+            out[name] = {"us": round(v["us"], 2), "rel_l2": v["rel_l2"]}
+            if not name.startswith("fused"):
+                st_us, _ = dominant_stage(torch, wc, v["op"], args)
+                out[name]["stage_us"] = {k: round(x, 2) for k, x in st_us.items()}
             v["op"].destroy()
+        print(json.dumps(out), flush=True)
         return
     resident = {k: v for k, v in conv.items() if v["resident"] and v["rel_l2"] <= 1e-2}
     chosen_name = min(resident, key=lambda k: resident[k]["us"])
