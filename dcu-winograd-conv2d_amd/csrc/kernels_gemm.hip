@@ -43,7 +43,7 @@ struct TileShape {
 
 // One BK x BP tile of M[t] over reduction steps [s0, s1) (32 channels each).
 template <int WK, int WP, int TK, int TP>
-__device__ inline void gemm_tile_f16(char* smem, const half8* __restrict__ U,
+__device__ __attribute__((always_inline)) inline void gemm_tile_f16(char* smem, const half8* __restrict__ U,
                                      const half8* __restrict__ V, half_t* __restrict__ M,
                                      float* __restrict__ M32, uint32_t Kp, uint64_t Pp,
                                      uint32_t C8p, uint32_t T2, uint32_t t, uint32_t kb,
@@ -151,7 +151,7 @@ __device__ inline void gemm_tile_f16(char* smem, const half8* __restrict__ U,
 // fp8 operands: 32-byte groups [t][c32][rows][32]; PLANES 1 (e4m3) or 2
 // (hi plane followed by lo plane, each T2*C32p*rows*32 bytes).
 template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
-__device__ inline void gemm_tile_f8(char* smem, const intx8* __restrict__ U,
+__device__ __attribute__((always_inline)) inline void gemm_tile_f8(char* smem, const intx8* __restrict__ U,
                                     const intx8* __restrict__ V, half_t* __restrict__ M,
                                     float* __restrict__ M32, const float* __restrict__ us,
                                     const float* __restrict__ vs, uint32_t Kp, uint64_t Pp,

@@ -53,7 +53,7 @@ __host__ __device__ inline uint32_t in_lds_bytes(int m, int CG, uint32_t nth, ui
 }
 
 template <int m, int CG, int PREC, bool EVENW>
-__device__ inline void input_block(const half_t* __restrict__ X, void* __restrict__ Vout,
+__device__ __attribute__((always_inline)) inline void input_block(const half_t* __restrict__ X, void* __restrict__ Vout,
                                    const InputGeom& g, const float* __restrict__ amax,
                                    float* __restrict__ vscale, uint32_t bx, uint32_t cg,
                                    half_t* xs, bool write_scale) {
@@ -86,35 +86,46 @@ __device__ inline void input_block(const half_t* __restrict__ X, void* __restric
         (void*)(X + ((uint64_t)n * g.C + (nch ? c0 : 0u)) * plane), 0,
         (int)(uint32_t)(nch * plane * 2u), 0x00020000);
     uint32_t* xs32 = (uint32_t*)xs;
-    const int R = CG * L.LH;
+    // Row r = ch * LH + lr of the staged image advances by rpp rows per
+    // load; the global byte offset (32-bit: the buffer range is < 2^32, and
+    // offsets of masked rows may wrap harmlessly) and the LDS dword index
+    // advance by constants, with one correction when lr wraps past LH.
+    const uint32_t plane2 = (uint32_t)plane * 2u, W2 = g.W * 2u;
     const int dch = rpp / L.LH, dlr = rpp % L.LH;
+    const uint32_t g_step = (uint32_t)dch * plane2 + (uint32_t)dlr * W2;
+    const uint32_t g_wrap = plane2 - (uint32_t)L.LH * W2;
+    const int l_step = (dch * L.PS + dlr * L.LWp) / 2, l_wrap = (L.PS - L.LH * L.LWp) / 2;
+    const int lr_lo = y0 < 0 ? -y0 : 0, lr_hi = min(L.LH, (int)g.H - y0);
     if (slot < rpp) {
-      int r = slot, ch = slot / L.LH, lr = slot % L.LH;
-      while (r < R) {
+      int ch = slot / L.LH, lr = slot % L.LH;
+      uint32_t goff = (uint32_t)ch * plane2 + (uint32_t)(y0 + lr) * W2 + 2u * (uint32_t)x;
+      int loff = (ch * L.PS + lr * L.LWp) / 2 + dcol;
+      while (ch < CG) {
         uint32_t val[SB];
         int dst[SB];
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
-          const int y = y0 + lr;
-          const bool iny = r < R && (uint32_t)ch < nch && y >= 0 && y < (int)g.H;
-          const uint32_t e = (uint32_t)(((uint64_t)ch * plane + (uint64_t)(y < 0 ? 0 : y) * g.W) * 2u);
+          const bool iny = (uint32_t)ch < nch && lr >= lr_lo && lr < lr_hi;
           if constexpr (EVENW) {
             const bool in = iny && x0in;
-            const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rs, in ? e + 2u * x : 0u, 0, 0);
+            const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rs, in ? goff : 0u, 0, 0);
             val[u] = in ? v : 0u;
           } else {
             const bool in0 = iny && x0in, in1 = iny && x1in;
-            const uint32_t a = __builtin_amdgcn_raw_buffer_load_b16(rs, in0 ? e + 2u * x : 0u, 0, 0);
-            const uint32_t c = __builtin_amdgcn_raw_buffer_load_b16(rs, in1 ? e + 2u * x + 2u : 0u, 0, 0);
+            const uint32_t a = __builtin_amdgcn_raw_buffer_load_b16(rs, in0 ? goff : 0u, 0, 0);
+            const uint32_t c = __builtin_amdgcn_raw_buffer_load_b16(rs, in1 ? goff + 2u : 0u, 0, 0);
             val[u] = (in0 ? (a & 0xffffu) : 0u) | ((in1 ? (c & 0xffffu) : 0u) << 16);
           }
-          dst[u] = r < R ? (ch * L.PS + lr * L.LWp) / 2 + dcol : -1;
-          r += rpp;
+          dst[u] = ch < CG ? loff : -1;
           ch += dch;
           lr += dlr;
+          goff += g_step;
+          loff += l_step;
           if (lr >= L.LH) {
             lr -= L.LH;
             ++ch;
+            goff += g_wrap;
+            loff += l_wrap;
           }
         }
 #pragma unroll
@@ -224,7 +235,7 @@ __global__ __launch_bounds__(256) void k_input(const half_t* __restrict__ X, voi
 // With split > 1 the GEMM left `split` FP32 partial sums [split][T2][Kp][Pp]
 // in M32; they are added in a fixed order (deterministic).
 template <int m>
-__device__ inline void output_pair(const half_t* __restrict__ M, const float* __restrict__ M32,
+__device__ __attribute__((always_inline)) inline void output_pair(const half_t* __restrict__ M, const float* __restrict__ M32,
                                    int split, half_t* __restrict__ Y, uint32_t K, uint32_t TH,
                                    uint32_t TW, uint64_t P, uint64_t Pp, uint32_t Kp,
                                    uint32_t OH, uint32_t OW, uint64_t idx) {
@@ -290,13 +301,97 @@ __device__ inline void output_pair(const half_t* __restrict__ M, const float* __
   }
 }
 
+// Stores the m x m patch y of tile p (output channel k), masked at the image
+// edge; whole rows as one vector store when the row is aligned and in range.
+template <int m>
+__device__ __attribute__((always_inline)) inline void store_patch(half_t* __restrict__ Y, const float (&y)[m * m], uint32_t K,
+                                   uint32_t k, uint32_t TH, uint32_t TW, uint32_t OH, uint32_t OW,
+                                   uint64_t p) {
+  const uint32_t per_img = TH * TW;
+  const uint32_t n = (uint32_t)(p / per_img);
+  const uint32_t rem = (uint32_t)(p - (uint64_t)n * per_img);
+  const uint32_t th = rem / TW, tw = rem - th * TW;
+  half_t* dst = Y + (((uint64_t)n * K + k) * OH + th * m) * OW + tw * m;
+  const bool full_w = tw * m + m <= OW && (OW % m) == 0;
+#pragma unroll
+  for (int i = 0; i < m; ++i) {
+    if (th * m + i >= OH) break;
+    if (full_w) {
+      if (m == 4) {
+        *(half4*)(dst + i * OW) = half4{(half_t)y[i * m], (half_t)y[i * m + 1],
+                                        (half_t)y[i * m + 2], (half_t)y[i * m + 3]};
+      } else {
+        *(half2_t*)(dst + i * OW) = half2_t{(half_t)y[i * m], (half_t)y[i * m + 1]};
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < m; ++j)
+        if (tw * m + j < OW) dst[i * OW + j] = (half_t)y[i * m + j];
+    }
+  }
+}
+
+// LDS-staged output transform of one output channel k and OUT_TB<m> tiles
+// starting at p0: the block first copies M[t][k][p0 .. p0+TB) for all T*T
+// positions into LDS with 16-byte loads (a wave reads 1 KB runs of M, every
+// thread issues all its loads before the first LDS write), then each thread
+// transforms TB/256 tiles reading its T*T values from LDS.
+template <int m>
+struct OutTB {
+  static constexpr int TB = m == 4 ? 256 : 512;  // tiles per block
+  static constexpr int T2 = (m + 2) * (m + 2);
+  static constexpr int LDS = T2 * TB * 2;         // bytes
+};
+
+template <int m>
+__device__ __attribute__((always_inline)) inline void output_block(const half_t* __restrict__ M, half_t* __restrict__ Y,
+                                    uint32_t K, uint32_t TH, uint32_t TW, uint64_t P,
+                                    uint64_t Pp, uint32_t Kp, uint32_t OH, uint32_t OW,
+                                    uint32_t k, uint64_t p0, half_t* sm) {
+  constexpr int T2 = OutTB<m>::T2, TB = OutTB<m>::TB;
+  constexpr int CPR = TB / 8;                      // 16-byte chunks per position row
+  constexpr int NCH = (T2 * CPR + 255) / 256;      // chunks per thread
+  const int tid = threadIdx.x;
+  const uint64_t plane = (uint64_t)Kp * Pp;
+  const half_t* src = M + (uint64_t)k * Pp + p0;
+  uint4 r[NCH];
+#pragma unroll
+  for (int j = 0; j < NCH; ++j) {
+    const int q = tid + j * 256;
+    const int t = q / CPR, c = q - t * CPR;
+    // chunks past the padded tile count (last block) or past T2 rows are not
+    // loaded; Pp is a multiple of 16, so a chunk is all in or all out
+    const bool ok = q < T2 * CPR && p0 + (uint64_t)c * 8 < Pp;
+    r[j] = ok ? *(const uint4*)(src + t * plane + c * 8) : uint4{0u, 0u, 0u, 0u};
+  }
+#pragma unroll
+  for (int j = 0; j < NCH; ++j) {
+    const int q = tid + j * 256;
+    if (q < T2 * CPR) *(uint4*)(sm + q * 8) = r[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < TB / 256; ++h) {
+    const int i = tid + h * 256;
+    const uint64_t p = p0 + i;
+    if (p < P) {
+      float x[T2], y[m * m];
+#pragma unroll
+      for (int t = 0; t < T2; ++t) x[t] = (float)sm[t * TB + i];
+      output2d<m>(x, y);
+      store_patch<m>(Y, y, K, k, TH, TW, OH, OW, p);
+    }
+  }
+}
+
 template <int m>
 __global__ __launch_bounds__(256) void k_output(const half_t* __restrict__ M,
                                                 half_t* __restrict__ Y, uint32_t K, uint32_t TH,
                                                 uint32_t TW, uint64_t P, uint64_t Pp, uint32_t Kp,
-                                                uint32_t OH, uint32_t OW) {
-  output_pair<m>(M, nullptr, 1, Y, K, TH, TW, P, Pp, Kp, OH, OW,
-                 (uint64_t)blockIdx.x * 256u + threadIdx.x);
+                                                uint32_t OH, uint32_t OW, uint32_t pblocks) {
+  __shared__ __attribute__((aligned(16))) half_t sm[OutTB<m>::LDS / 2];
+  const uint32_t k = blockIdx.x / pblocks, pb = blockIdx.x - k * pblocks;
+  output_block<m>(M, Y, K, TH, TW, P, Pp, Kp, OH, OW, k, (uint64_t)pb * OutTB<m>::TB, sm);
 }
 
 // --------------------------------------------------------------- geometry
@@ -374,13 +469,14 @@ hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N,
 hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N, uint32_t K,
                                    _Float16* y, hipStream_t st) {
   (void)N;
-  dim3 grid(div_up((uint64_t)K * ((pl.P + 1) / 2), 256));
+  const uint32_t pblocks = div_up(pl.P, pl.m == 4 ? OutTB<4>::TB : OutTB<2>::TB);
+  dim3 grid(pblocks * K);
   if (pl.m == 2)
     hipLaunchKernelGGL(k_output<2>, grid, dim3(256), 0, st, M, y, K, pl.tiles_h, pl.tiles_w, pl.P,
-                       pl.Pp, pl.Kp, pl.oh, pl.ow);
+                       pl.Pp, pl.Kp, pl.oh, pl.ow, pblocks);
   else
     hipLaunchKernelGGL(k_output<4>, grid, dim3(256), 0, st, M, y, K, pl.tiles_h, pl.tiles_w, pl.P,
-                       pl.Pp, pl.Kp, pl.oh, pl.ow);
+                       pl.Pp, pl.Kp, pl.oh, pl.ow, pblocks);
   return hipGetLastError();
 }
 

@@ -61,13 +61,19 @@ struct Tf<4> {
     z[4] = a - g[1] * (1.0f / 12.0f);
     z[5] = g[2];
   }
+  // 12 operations: the row pairs (1, 2) and (3, 4) share their even and
+  // odd parts
   __device__ static inline void BT(const float* d, float* z) {
-    z[0] = 4.0f * d[0] - 5.0f * d[2] + d[4];
-    z[1] = -4.0f * (d[1] + d[2]) + d[3] + d[4];
-    z[2] = 4.0f * (d[1] - d[2]) - d[3] + d[4];
-    z[3] = -2.0f * (d[1] - d[3]) - d[2] + d[4];
-    z[4] = 2.0f * (d[1] - d[3]) - d[2] + d[4];
-    z[5] = 4.0f * d[1] - 5.0f * d[3] + d[5];
+    const float p = __builtin_fmaf(-4.0f, d[2], d[4]);  // d4 - 4 d2
+    const float q = __builtin_fmaf(-4.0f, d[1], d[3]);  // d3 - 4 d1
+    const float e = d[4] - d[2];
+    const float o = d[3] - d[1];
+    z[0] = __builtin_fmaf(4.0f, d[0], __builtin_fmaf(-5.0f, d[2], d[4]));
+    z[1] = p + q;
+    z[2] = p - q;
+    z[3] = __builtin_fmaf(2.0f, o, e);
+    z[4] = __builtin_fmaf(-2.0f, o, e);
+    z[5] = __builtin_fmaf(4.0f, d[1], __builtin_fmaf(-5.0f, d[3], d[5]));
   }
   __device__ static inline void AT(const float* x, float* y) {
     const float a = x[1] + x[2], b = x[1] - x[2], c = x[3] + x[4], d = x[3] - x[4];
