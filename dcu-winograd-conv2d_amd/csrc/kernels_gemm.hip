@@ -148,6 +148,110 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16(char* smem, 
   __syncthreads();  // LDS is reused by the caller's next tile
 }
 
+// 256 x 64 tile (4 waves, each owning 64 rows of K) whose A fragments
+// bypass LDS: in the [t][c8][k][8] layout a wave's 16 x 32 fragment is four
+// 256-byte runs of U, so each wave loads its own rows straight into MFMA
+// operand registers; only B (shared by the 4 waves) is staged through LDS.
+// That cuts LDS traffic per step from 52 KB to 20 KB per block.  Loads run
+// ahead through register rings (A: 2 steps, B: 3 steps) across LDS-only
+// barriers, so global loads stay in flight while the MFMAs of earlier steps
+// run; slot indices are static because the loop is unrolled by 6.
+__device__ __attribute__((always_inline)) inline void gemm_tile_f16_rows(
+    char* smem, const half8* __restrict__ U, const half8* __restrict__ V, half_t* __restrict__ M,
+    float* __restrict__ M32, uint32_t Kp, uint64_t Pp, uint32_t C8p, uint32_t T2, uint32_t t,
+    uint32_t kb, uint64_t pb, int s0, int s1, int sp) {
+  constexpr int TK = 4, TP = 4, BK = 256, BP = 64, CS = BP + 8;
+  half8* Bs = (half8*)smem;  // [2][4][BP]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int g = lane >> 4, r16 = lane & 15;
+  const uint32_t k0 = kb * BK;
+  const uint64_t p0 = pb * BP;
+  const half8* Ua = U + (uint64_t)t * C8p * Kp + (uint64_t)g * Kp + k0 + wave * 64 + r16;
+  const half8* Vt = V + (uint64_t)t * C8p * Pp + p0;
+  const int bq_g = tid / BP, bq_c = tid % BP;  // this thread's B chunk (one per step)
+  const half8* Vb = Vt + (uint64_t)bq_g * Pp + bq_c;
+
+  auto load_a = [&](int s, half8 (&a)[TK]) {
+    s = s < s1 ? s : s1 - 1;
+    const half8* src = Ua + (uint64_t)(4 * s) * Kp;
+#pragma unroll
+    for (int i = 0; i < TK; ++i) a[i] = src[i * 16];
+  };
+  auto load_b = [&](int s, half8& b) {
+    s = s < s1 ? s : s1 - 1;
+    b = Vb[(uint64_t)(4 * s) * Pp];
+  };
+
+  floatx4 acc[TK][TP];
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  half8 ra[2][TK], rb[3];
+  // prologue: A(s0), B(s0), B(s0+1), A(s0+1), B(s0+2) -- the per-step issue
+  // order below is A(s+2), B(s+3)
+  load_a(s0, ra[0]);
+  load_b(s0, rb[0]);
+  load_b(s0 + 1, rb[1]);
+  load_a(s0 + 1, ra[1]);
+  load_b(s0 + 2, rb[2]);
+  Bs[tid] = rb[0];
+  load_b(s0 + 3, rb[0]);
+  lds_barrier();
+
+  auto step = [&](int s, half8 (&a)[TK], half8& bnext, half8& bfree) {
+    (void)bfree;
+    const int cur = (s - s0) & 1;
+    half8 b[TP];
+#pragma unroll
+    for (int j = 0; j < TP; ++j) b[j] = Bs[(cur * 4 + g) * BP + j * 16 + r16];
+#pragma unroll
+    for (int i = 0; i < TK; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+    Bs[(cur ^ 1) * 4 * BP + tid] = bnext;  // B(s+1); unused after the last step
+    load_a(s + 2, a);                       // a held A(s): refill with A(s+2)
+    load_b(s + 4, bnext);                   // bnext held B(s+1): refill with B(s+4)
+    lds_barrier();
+  };
+  // slots: A(s) in ra[s % 2], B(s) in rb[s % 3] (relative to s0)
+  for (int base = s0; base < s1; base += 6) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (base + d < s1) step(base + d, ra[d % 2], rb[(d + 1) % 3], rb[d % 3]);
+  }
+
+  if (M32) {
+    float* Mp = M32 + (((uint64_t)sp * T2 + t) * Kp + k0) * Pp + p0;
+#pragma unroll
+    for (int i = 0; i < TK; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          Mp[(uint64_t)(wave * 64 + i * 16 + 4 * g + e) * Pp + j * 16 + r16] = acc[i][j][e];
+    return;
+  }
+  half_t* Cs = (half_t*)smem;
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int j = 0; j < TP; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        Cs[(wave * 64 + i * 16 + 4 * g + e) * CS + j * 16 + r16] = (half_t)acc[i][j][e];
+  lds_barrier();
+  half_t* Mt = M + ((uint64_t)t * Kp + k0) * Pp + p0;
+  for (int q = tid; q < BK * BP / 8; q += 256) {
+    const int row = q / (BP / 8), c8 = q % (BP / 8);
+    *(half8*)(Mt + (uint64_t)row * Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
+  }
+  lds_barrier();  // LDS is reused by the caller's next tile
+}
+
 // fp8 operands: 32-byte groups [t][c32][rows][32]; PLANES 1 (e4m3) or 2
 // (hi plane followed by lo plane, each T2*C32p*rows*32 bytes).
 template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
@@ -311,8 +415,11 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
   __shared__ __attribute__((aligned(16))) char smem[TileShape<WK, WP, TK, TP>::lds_f16()];
   uint32_t pb, kb, t;
   tile_of_block(pb, kb, t);
-  gemm_tile_f16<WK, WP, TK, TP>(smem, U, V, M, nullptr, Kp, Pp, C8p, T2, t, kb, pb, 0,
-                                (int)(C8p / 4), 0);
+  if constexpr (WK == 4 && WP == 1 && TK == 4 && TP == 4)
+    gemm_tile_f16_rows(smem, U, V, M, nullptr, Kp, Pp, C8p, T2, t, kb, pb, 0, (int)(C8p / 4), 0);
+  else
+    gemm_tile_f16<WK, WP, TK, TP>(smem, U, V, M, nullptr, Kp, Pp, C8p, T2, t, kb, pb, 0,
+                                  (int)(C8p / 4), 0);
 }
 
 template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
@@ -447,7 +554,7 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __r
       case 1: gemm_tile_f16<2, 2, 2, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
       case 2: gemm_tile_f16<2, 2, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
       case 3: gemm_tile_f16<1, 4, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-      case 5: gemm_tile_f16<4, 1, 4, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
+      case 5: gemm_tile_f16_rows(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
       default: gemm_tile_f16<1, 4, 1, 1>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
     }
   }

@@ -154,6 +154,15 @@ __device__ inline void output2d(const float* x, float* y) {
   }
 }
 
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// accesses (lgkmcnt) but not for its outstanding global loads, which may
+// stay in flight across the barrier (__syncthreads() would drain them).
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // OCP FP8 E4M3 (bias 7, max 448, no infinities): round to nearest even,
 // saturating to +-448.  Written out in integer arithmetic so that the host
 // oracle's quantizer and this one agree bit for bit.
