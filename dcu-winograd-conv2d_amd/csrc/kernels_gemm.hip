@@ -207,11 +207,13 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16_rows(
     half8 b[TP];
 #pragma unroll
     for (int j = 0; j < TP; ++j) b[j] = Bs[(cur * 4 + g) * BP + j * 16 + r16];
+    // transposed product (A operand = V, B operand = U): lane ends with 4
+    // consecutive tiles of one output channel
 #pragma unroll
     for (int i = 0; i < TK; ++i)
 #pragma unroll
       for (int j = 0; j < TP; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[j], a[i], acc[i][j], 0, 0, 0);
     Bs[(cur ^ 1) * 4 * BP + tid] = bnext;  // B(s+1); unused after the last step
     load_a(s + 2, a);                       // a held A(s): refill with A(s+2)
     load_b(s + 4, bnext);                   // bnext held B(s+1): refill with B(s+4)
@@ -224,15 +226,14 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16_rows(
       if (base + d < s1) step(base + d, ra[d % 2], rb[(d + 1) % 3], rb[d % 3]);
   }
 
+  // acc[i][j][e]: channel k0 + 64 wave + 16 i + r16, tile p0 + 16 j + 4 g + e
   if (M32) {
     float* Mp = M32 + (((uint64_t)sp * T2 + t) * Kp + k0) * Pp + p0;
 #pragma unroll
     for (int i = 0; i < TK; ++i)
 #pragma unroll
       for (int j = 0; j < TP; ++j)
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          Mp[(uint64_t)(wave * 64 + i * 16 + 4 * g + e) * Pp + j * 16 + r16] = acc[i][j][e];
+        *(floatx4*)(Mp + (uint64_t)(wave * 64 + i * 16 + r16) * Pp + j * 16 + 4 * g) = acc[i][j];
     return;
   }
   half_t* Cs = (half_t*)smem;
@@ -240,9 +241,9 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16_rows(
   for (int i = 0; i < TK; ++i)
 #pragma unroll
     for (int j = 0; j < TP; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        Cs[(wave * 64 + i * 16 + 4 * g + e) * CS + j * 16 + r16] = (half_t)acc[i][j][e];
+      *(half4*)(Cs + (wave * 64 + i * 16 + r16) * CS + j * 16 + 4 * g) =
+          half4{(half_t)acc[i][j][0], (half_t)acc[i][j][1], (half_t)acc[i][j][2],
+                (half_t)acc[i][j][3]};
   lds_barrier();
   half_t* Mt = M + ((uint64_t)t * Kp + k0) * Pp + p0;
   for (int q = tid; q < BK * BP / 8; q += 256) {
