@@ -372,10 +372,9 @@ def main():
         a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda").fill_(1)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": a.numel(), "copies": 5}))
+        ms = time_loop(torch, lambda: b.copy_(a), 5, 2)
+        print(json.dumps({"copy_bytes": a.numel(), "copies": 7, "ms_per_copy": round(ms, 4),
+                          "read_plus_write_GBps": round(2 * a.numel() / (ms * 1e-3) / 1e9, 1)}))
         return
     import winoconv as wc
 
