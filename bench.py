@@ -198,7 +198,8 @@ def bench_grouped(torch, wc, args, tuning):
     batch = wc.Batch(eng, 16)
     stream = torch.cuda.Stream()
     try:
-        fids = [eng.register_filter(d[3].data_ptr(), p[4], p[1]) for p, d in zip(probs, data)]
+        fids = [eng.register_filter(d[3].data_ptr(), p[4], p[1], wc.ALGO_LEAST_TRAFFIC)
+                for p, d in zip(probs, data)]
         for (n, c, h, w, k), fid, d in zip(probs, fids, data):
             batch.add(fid, d[2].data_ptr(), d[4].data_ptr(), n, h, w)
         algos = [batch.info(i).m for i in range(len(probs))]
@@ -249,7 +250,7 @@ def bench_serving(torch, wc, args, tuning):
         for (c, k, hw) in CLASSES:
             x, wt = he_inputs(rng, nmax, c, hw, hw, k)
             dx, dw = dev(torch, x), dev(torch, wt)
-            fid = eng.register_filter(dw.data_ptr(), k, c)
+            fid = eng.register_filter(dw.data_ptr(), k, c, wc.ALGO_LEAST_TRAFFIC)
             ring = [torch.empty((nmax, k, hw, hw), dtype=torch.float16, device="cuda")
                     for _ in range(32)]
             cls.append({"x": x, "wt": wt, "dx": dx, "dw": dw, "fid": fid, "ring": ring, "c": c,
@@ -270,7 +271,8 @@ def bench_serving(torch, wc, args, tuning):
                                    dtype=torch.float16, device="cuda")
                     for r, (ci, n, _) in enumerate(plan) if r % 100 == 0}
             torch.cuda.synchronize()
-            srv = wc.Server(eng, slots=args.slots, max_batch=args.max_batch, use_graph=True)
+            srv = wc.Server(eng, slots=args.slots, max_batch=args.max_batch,
+                            use_graph=not args.no_graph)
             reqs, checks = [], []
             F = 0.0
             try:
@@ -314,6 +316,8 @@ def bench_serving(torch, wc, args, tuning):
                    "pool_peak_mib": round(st.pool_peak / 2**20, 1),
                    "mean_batch_gpu_us": round(st.mean_batch_gpu_us, 1),
                    "mean_queue_us": round(st.mean_queue_us, 1),
+                   "max_batch_gpu_us": round(st.max_batch_gpu_us, 1),
+                   "max_queue_us": round(st.max_queue_us, 1),
                    "checked": len(errs), "max_rel_l2": float(f"{max(errs):.3e}") if errs else None}
             if frac is None:
                 capacity = nreq / wall
@@ -360,6 +364,9 @@ def main():
     ap.add_argument("--grouped-only", action="store_true", help="run the grouped workload only")
     ap.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
     ap.add_argument("--max-batch", type=int, default=16, help="serving: requests per batch")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="serving: eager launches (only the kernels a batch needs) instead of "
+                         "replaying each slot's captured graph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (HBM counter calibration)")
     args = ap.parse_args()

@@ -53,7 +53,7 @@ def main():
             wt = torch.from_numpy((rng.standard_normal((k, c, 3, 3)) * 0.05).astype(np.float16)).cuda()
             fid = eng.register_filter(wt.data_ptr(), k, c)
             rows = []
-            for n in LADDER:
+            for n, algo in [(n, a) for n in LADDER for a in (wc.ALGO_F23, wc.ALGO_F43)]:
                 x = torch.randn((n, c, hw, hw), device="cuda").half()
                 copies = int(min(64, max(1, -(-2e10 // (2.0 * n * k * hw * hw * c * 9)))))
                 ys = [torch.empty((n, k, hw, hw), device="cuda", dtype=torch.float16)
@@ -62,7 +62,7 @@ def main():
                 b = wc.Batch(eng, copies)
                 try:
                     for y in ys:
-                        b.add(fid, x.data_ptr(), y.data_ptr(), n, hw, hw)
+                        b.add(fid, x.data_ptr(), y.data_ptr(), n, hw, hw, algo)
                     inf = b.info(0)
                     m, P = inf.m, inf.tiles
                     nsteps = inf.cp // (32 if args.precision == 0 else 128)

@@ -235,6 +235,12 @@ static int plan_batched(const wino_engine* e, uint32_t K, uint32_t C, uint32_t n
 
 static int filter_m(const wino_engine::Filter& f, int algo, uint32_t n, uint32_t h, uint32_t w) {
   int a = algo == WINO_ALGO_AUTO ? f.algo : algo;
+  if (a == WINO_ALGO_LEAST_TRAFFIC) {
+    // T^2 * tiles per channel of each tile size (V and M scale with it)
+    const uint64_t t2 = 16ull * ((h + 1) / 2) * ((w + 1) / 2);
+    const uint64_t t4 = 36ull * ((h + 3) / 4) * ((w + 3) / 4);
+    return t4 <= t2 ? 4 : 2;
+  }
   if (a == WINO_ALGO_AUTO) {
     problem_t pr;
     memset(&pr, 0, sizeof(pr));
@@ -282,15 +288,28 @@ struct wino_batch {
 
 namespace wino {
 
-static int batch_enqueue_kernels(wino_batch* b, hipStream_t st) {
+// Enqueues the batch's kernels.  `all`: every launch (graph capture, for any
+// later contents); otherwise only those with work in the current descriptors
+// (the host copy in `b->host` matches the device copy).
+static int batch_enqueue_kernels(wino_batch* b, hipStream_t st, bool all) {
   const GroupHeader* hdr = (const GroupHeader*)b->dev;
   const GProb* probs = (const GProb*)(b->dev + b->probs_off);
+  const GroupHeader* h = (const GroupHeader*)b->host;
+  uint32_t in_mask = 3u, gemm_mask = 0x3fu, out_mask = 7u;
+  if (!all) {
+    in_mask = (h->in_blocks[0] ? 1u : 0u) | (h->in_blocks[1] ? 2u : 0u);
+    out_mask = (h->out_blocks[0] ? 1u : 0u) | (h->out_blocks[1] ? 2u : 0u) |
+               (h->out_blocks[2] ? 4u : 0u);
+    gemm_mask = 0;
+    for (int c = 0; c < kGemmCfgs; ++c)
+      if (h->cfg_items[c]) gemm_mask |= 1u << c;
+  }
   const int prec = b->eng->precision;
   hipError_t e = hipSuccess;
   if (prec != WINO_PREC_FP16) e = launch_amax_grouped(hdr, probs, b->grid_amax, st);
-  if (e == hipSuccess) e = launch_input_grouped(prec, hdr, probs, b->grid_in, st);
-  if (e == hipSuccess) e = launch_gemm_grouped(prec, hdr, probs, b->grid_gemm, st);
-  if (e == hipSuccess) e = launch_output_grouped(hdr, probs, b->grid_out, st);
+  if (e == hipSuccess) e = launch_input_grouped(prec, hdr, probs, b->grid_in, in_mask, st);
+  if (e == hipSuccess) e = launch_gemm_grouped(prec, hdr, probs, b->grid_gemm, gemm_mask, st);
+  if (e == hipSuccess) e = launch_output_grouped(hdr, probs, b->grid_out, out_mask, st);
   return hipst(e);
 }
 
@@ -376,7 +395,7 @@ int wino_engine_register_filter(wino_engine* e, const void* weight, uint32_t k, 
                                 int algo, int* filter_id) {
   if (!e || !weight || !filter_id || !k || !c) return WINO_ERR_INVALID;
   if (algo != WINO_ALGO_AUTO && algo != WINO_ALGO_F23 && algo != WINO_ALGO_F43 &&
-      algo != WINO_ALGO_F23_FUSED)
+      algo != WINO_ALGO_F23_FUSED && algo != WINO_ALGO_LEAST_TRAFFIC)
     return WINO_ERR_INVALID;
   wino_engine::Filter f;
   f.weight = weight;
@@ -385,7 +404,8 @@ int wino_engine_register_filter(wino_engine* e, const void* weight, uint32_t k, 
   f.algo = algo;
   f.Kp = (uint32_t)ru(k, 128);
   f.Cp = (uint32_t)ru(c, e->precision == WINO_PREC_FP16 ? 32 : 128);
-  const bool need2 = algo != WINO_ALGO_F43, need4 = algo == WINO_ALGO_F43 || algo == WINO_ALGO_AUTO;
+  const bool both = algo == WINO_ALGO_AUTO || algo == WINO_ALGO_LEAST_TRAFFIC;
+  const bool need2 = algo != WINO_ALGO_F43, need4 = algo == WINO_ALGO_F43 || both;
   for (int i = 0; i < 2; ++i) {
     if (!(i ? need4 : need2)) continue;
     const int t2 = i ? 36 : 16;
@@ -473,7 +493,7 @@ int wino_batch_create(wino_engine* e, int max_problems, wino_batch** out) {
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus <= 0) cus = 256;
-  b->grid_gemm = (uint32_t)cus * 2;
+  b->grid_gemm = (uint32_t)cus * 4;
   b->grid_in = (uint32_t)cus * 4;
   b->grid_out = (uint32_t)cus * 8;
   b->grid_amax = (uint32_t)cus * 2;
@@ -524,7 +544,7 @@ static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, voi
   g.M32 = en.bp.split > 1 ? (float*)(ws + en.bp.m32_off) : nullptr;
   g.vs = e->precision == WINO_PREC_FP16 ? nullptr : (float*)(ws + en.bp.sc_off);
   g.amax = e->precision == WINO_PREC_FP16 ? nullptr : (float*)(ws + en.bp.sc_off + 256);
-  g.ig = input_geom(pl, n, f.C, h, w, 40u * 1024u);
+  g.ig = input_geom(pl, n, f.C, h, w, kGroupInputLds);
   g.P = pl.P;
   g.Pp = pl.Pp;
   g.K = f.K;
@@ -541,8 +561,14 @@ static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, voi
   // writes the zero channel groups of the padded reduction (c >= C); the
   // padded tile columns (p >= P) only reach GEMM columns nobody reads.
   g.in_gx = n * g.ig.nwr * g.ig.nwc;
-  g.in_count = g.in_gx * g.ig.CGp;
-  g.out_count = (uint32_t)(((uint64_t)f.K * ((pl.P + 1) / 2) + 255) / 256);
+  const int ms = pl.m == 4 ? 1 : 0;
+  g.in_count[ms] = g.in_gx * g.ig.CGp;
+  const uint32_t tb = pl.m == 4 ? out_tiles_per_block(4) : out_tiles_per_block(2);
+  g.out_pblocks = (uint32_t)((pl.P + tb - 1) / tb);
+  if (en.bp.split > 1)
+    g.out_count[2] = (uint32_t)(((uint64_t)f.K * ((pl.P + 1) / 2) + 255) / 256);
+  else
+    g.out_count[ms] = f.K * g.out_pblocks;
   const uint64_t nin = (uint64_t)n * f.C * h * w;
   g.amax_count = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, nin / (256 * 64)));
 }
@@ -632,8 +658,10 @@ int wino_batch_info(wino_batch* b, int index, wino_info_t* info) {
 }
 
 // Writes the header and the problem descriptors into the pinned staging
-// buffer, problems in decreasing cost per GEMM item (LPT order for the
-// persistent GEMM's round-robin); returns the used size.
+// buffer; returns the used size.  Problems are grouped by GEMM tile
+// configuration (one grouped GEMM launch per configuration walks its own
+// item range) and, within one, ordered by decreasing cost per item (LPT
+// order for the persistent round-robin).
 static size_t build_descriptors(wino_batch* b) {
   GroupHeader* hdr = (GroupHeader*)b->host;
   GProb* probs = (GProb*)(b->host + b->probs_off);
@@ -645,29 +673,38 @@ static size_t build_descriptors(wino_batch* b) {
     order[i] = (uint32_t)i;
     cost[i] = (uint64_t)bp.pl.bk * bp.pl.bp * (b->entries[i].gp.nsteps / bp.split);
   }
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
-  uint32_t in_first = 0, out_first = 0, amax_first = 0;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+    const int ca = b->entries[a].bp.pl.gemm_cfg, cc = b->entries[c].bp.pl.gemm_cfg;
+    return ca != cc ? ca < cc : cost[a] > cost[c];
+  });
+  uint32_t in_first[2] = {0, 0}, out_first[3] = {0, 0, 0}, amax_first = 0;
   uint64_t item_first = 0;
   for (size_t j = 0; j < order.size(); ++j) {
     const auto& en = b->entries[order[j]];
     GProb g = en.gp;
-    g.in_first = in_first;
-    in_first += g.in_count;
-    g.out_first = out_first;
-    out_first += g.out_count;
+    for (int q = 0; q < 2; ++q) {
+      g.in_first[q] = in_first[q];
+      in_first[q] += g.in_count[q];
+    }
+    for (int q = 0; q < 3; ++q) {
+      g.out_first[q] = out_first[q];
+      out_first[q] += g.out_count[q];
+    }
     g.amax_first = amax_first;
     amax_first += g.amax_count;
     g.kblocks = en.bp.kblocks;
     g.pblocks = (uint32_t)(en.bp.pl.Pp / en.bp.pl.bp);
     g.item_first = (uint32_t)item_first;
     g.item_count = (uint32_t)en.bp.items;
+    const int c = en.bp.pl.gemm_cfg;
+    if (hdr->cfg_items[c] == 0) hdr->cfg_first[c] = (uint32_t)item_first;
+    hdr->cfg_items[c] += (uint32_t)en.bp.items;
     item_first += en.bp.items;
     probs[j] = g;
   }
   hdr->nprob = (uint32_t)b->entries.size();
-  hdr->in_blocks = in_first;
-  hdr->out_blocks = out_first;
+  for (int q = 0; q < 2; ++q) hdr->in_blocks[q] = in_first[q];
+  for (int q = 0; q < 3; ++q) hdr->out_blocks[q] = out_first[q];
   hdr->nitems = (uint32_t)item_first;  // < 2^31: checked as problems are added
   hdr->amax_blocks = amax_first;
   return b->probs_off + order.size() * sizeof(GProb);
@@ -689,7 +726,7 @@ int wino_batch_launch(wino_batch* b, void* stream) {
     b->last_stream = st;
   }
   if (b->exec) return hipst(hipGraphLaunch(b->exec, st));
-  return batch_enqueue_kernels(b, st);
+  return batch_enqueue_kernels(b, st, false);
 }
 
 int wino_batch_capture(wino_batch* b, void* stream) {
@@ -706,7 +743,7 @@ int wino_batch_capture(wino_batch* b, void* stream) {
   }
   int rc = hipst(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   if (rc) return rc;
-  rc = batch_enqueue_kernels(b, st);
+  rc = batch_enqueue_kernels(b, st, true);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (rc) {
@@ -781,6 +818,7 @@ static void server_loop(wino_server* s) {
       {
         std::lock_guard<std::mutex> lk(s->mu);
         s->gpu_us_sum += 1e3 * ms;
+        s->stats.max_batch_gpu_us = std::max(s->stats.max_batch_gpu_us, 1e3 * (double)ms);
         for (uint64_t id : sl.reqs) {
           auto it = s->reqs.find(id);
           if (it != s->reqs.end()) {
@@ -800,25 +838,39 @@ static void server_loop(wino_server* s) {
     // the workspace pool cannot hold yet)
     for (auto& sl : s->slots) {
       if (sl.busy) continue;
-      std::vector<uint64_t> taken;
+      // take candidates under the lock, plan them into the batch without it
+      // (submitters are not blocked while problems are planned), and give
+      // back, in order, those the pool cannot hold yet
+      std::vector<std::pair<uint64_t, wino_server::Req>> cand;
       {
         std::lock_guard<std::mutex> lk(s->mu);
-        while (!s->pending.empty() && (int)taken.size() < s->max_batch) {
+        while (!s->pending.empty() && (int)cand.size() < s->max_batch) {
           const uint64_t id = s->pending.front();
-          const auto& r = s->reqs[id];
-          const int st = wino_batch_add(sl.batch, r.filter, r.in, r.out, r.n, r.h, r.w,
-                                        WINO_ALGO_AUTO, nullptr);
-          if (st == WINO_ERR_CAPACITY) break;
           s->pending.pop_front();
-          if (st) {  // cannot run: complete it with its error
-            s->reqs[id].done = true;
-            s->reqs[id].status = st;
-            s->reqs[id].t_done = now_us();
-            ++s->stats.rejected;
-            ++s->stats.completed;
-            continue;
-          }
-          taken.push_back(id);
+          cand.emplace_back(id, s->reqs[id]);
+        }
+      }
+      std::vector<uint64_t> taken;
+      size_t used = 0;
+      std::vector<std::pair<uint64_t, int>> failed;
+      for (; used < cand.size(); ++used) {
+        const auto& r = cand[used].second;
+        const int st = wino_batch_add(sl.batch, r.filter, r.in, r.out, r.n, r.h, r.w,
+                                      WINO_ALGO_AUTO, nullptr);
+        if (st == WINO_ERR_CAPACITY) break;
+        if (st) failed.emplace_back(cand[used].first, st);
+        else taken.push_back(cand[used].first);
+      }
+      if (used < cand.size() || !failed.empty()) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (size_t q = cand.size(); q > used; --q) s->pending.push_front(cand[q - 1].first);
+        for (const auto& f : failed) {  // cannot run: complete it with its error
+          auto& r = s->reqs[f.first];
+          r.done = true;
+          r.status = f.second;
+          r.t_done = now_us();
+          ++s->stats.rejected;
+          ++s->stats.completed;
         }
       }
       s->cv.notify_all();  // requests completed with an error wake drain()
@@ -838,7 +890,11 @@ static void server_loop(wino_server* s) {
       progressed = true;
       const double tl = now_us();
       std::lock_guard<std::mutex> lk(s->mu);
-      for (uint64_t id : taken) s->queue_us_sum += tl - s->reqs[id].t_submit;
+      for (uint64_t id : taken) {
+        const double q = tl - s->reqs[id].t_submit;
+        s->queue_us_sum += q;
+        s->stats.max_queue_us = std::max(s->stats.max_queue_us, q);
+      }
       s->launched += taken.size();
       ++s->stats.batches;
       s->stats.max_batch_problems = std::max<uint32_t>(s->stats.max_batch_problems, (uint32_t)taken.size());

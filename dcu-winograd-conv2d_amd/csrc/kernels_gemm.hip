@@ -446,7 +446,15 @@ static const int kCfg[][4] = {
     {1, 4, 1, 1},  // 4:  16 x  64
     {4, 1, 4, 4},  // 5: 256 x  64 (fp16 only: all of K <= 256 in one tile, V read once)
 };
+template <int CFG>
+struct kCfgT {
+  static constexpr int WK = CFG == 0 ? 2 : CFG == 1 ? 2 : CFG == 2 ? 2 : CFG == 3 ? 1 : CFG == 4 ? 1 : 4;
+  static constexpr int WP = CFG == 0 ? 2 : CFG == 1 ? 2 : CFG == 2 ? 2 : CFG == 3 ? 4 : CFG == 4 ? 4 : 1;
+  static constexpr int TK = CFG == 0 ? 4 : CFG == 1 ? 2 : CFG == 2 ? 2 : CFG == 3 ? 2 : CFG == 4 ? 1 : 4;
+  static constexpr int TP = CFG == 0 ? 4 : CFG == 1 ? 4 : CFG == 2 ? 2 : CFG == 3 ? 2 : CFG == 4 ? 1 : 4;
+};
 int gemm_cfg_count() { return (int)(sizeof(kCfg) / sizeof(kCfg[0])); }
+static_assert(sizeof(kCfg) / sizeof(kCfg[0]) == kGemmCfgs, "tile table size");
 bool gemm_cfg_ok(int idx, int precision) {
   return idx >= 0 && idx < gemm_cfg_count() && (precision == WINO_PREC_FP16 || idx < 5);
 }
@@ -536,13 +544,17 @@ __device__ inline GWork decode_item(const GProb& pr, uint32_t i) {
   return w;
 }
 
+// One launch per tile configuration: each kernel walks only the items of its
+// configuration, so its register and LDS budget is that tile's alone.
+template <int CFG>
 __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __restrict__ hdr,
                                                           const GProb* __restrict__ probs) {
   extern __shared__ __attribute__((aligned(16))) char gsmem[];
-  const uint32_t n = hdr->nitems, np = hdr->nprob;
-  ProbCursor<&GProb::item_first> cur;
+  const uint32_t first = hdr->cfg_first[CFG], end = first + hdr->cfg_items[CFG];
+  const uint32_t np = hdr->nprob;
+  ProbCursor<ItemFirst> cur;
   cur.init(probs, np);
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+  for (uint32_t i = first + blockIdx.x; i < end; i += gridDim.x) {
     cur.seek(probs, np, i);
     const GProb& pr = probs[cur.p];
     const GWork it = decode_item(pr, i);
@@ -550,25 +562,24 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __r
     float* m32 = pr.split > 1 ? pr.M32 : nullptr;
     const half8* U = (const half8*)pr.U;
     const half8* V = (const half8*)pr.V;
-    switch (pr.cfg) {
-      case 0: gemm_tile_f16<2, 2, 4, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-      case 1: gemm_tile_f16<2, 2, 2, 4>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-      case 2: gemm_tile_f16<2, 2, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-      case 3: gemm_tile_f16<1, 4, 2, 2>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-      case 5: gemm_tile_f16_rows(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-      default: gemm_tile_f16<1, 4, 1, 1>(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp); break;
-    }
+    if constexpr (CFG == 5)
+      gemm_tile_f16_rows(gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp);
+    else
+      gemm_tile_f16<kCfgT<CFG>::WK, kCfgT<CFG>::WP, kCfgT<CFG>::TK, kCfgT<CFG>::TP>(
+          gsmem, U, V, pr.M, m32, pr.Kp, pr.Pp, pr.Cp / 8, pr.t2, it.t, it.kb, it.pb, s0, s1, it.sp);
   }
 }
 
-template <int PLANES>
+template <int CFG, int PLANES>
 __global__ __launch_bounds__(256) void k_gemm_grouped_f8(const GroupHeader* __restrict__ hdr,
                                                          const GProb* __restrict__ probs) {
   extern __shared__ __attribute__((aligned(16))) char gsmem[];
-  const uint32_t n = hdr->nitems, np = hdr->nprob;
-  ProbCursor<&GProb::item_first> cur;
+  const uint32_t first = hdr->cfg_first[CFG], end = first + hdr->cfg_items[CFG];
+  const uint32_t np = hdr->nprob;
+  ProbCursor<ItemFirst> cur;
   cur.init(probs, np);
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+  using C = kCfgT<CFG>;
+  for (uint32_t i = first + blockIdx.x; i < end; i += gridDim.x) {
     cur.seek(probs, np, i);
     const GProb& pr = probs[cur.p];
     const GWork it = decode_item(pr, i);
@@ -577,59 +588,77 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f8(const GroupHeader* __re
     const intx8* U = (const intx8*)pr.U;
     const intx8* V = (const intx8*)pr.V;
     const uint32_t c32 = pr.Cp / 32;
-#define WINO_F8_TILE(WK, WP, TK, TP)                                                            \
-  do {                                                                                          \
-    if (PLANES == 2 && pr.m == 4)                                                               \
-      gemm_tile_f8<WK, WP, TK, TP, PLANES, true>(gsmem, U, V, pr.M, m32, pr.us, pr.vs, pr.Kp,   \
-                                                 pr.Pp, c32, pr.t2, it.t, it.kb, it.pb, s0, s1, \
-                                                 it.sp);                                        \
-    else                                                                                        \
-      gemm_tile_f8<WK, WP, TK, TP, PLANES, false>(gsmem, U, V, pr.M, m32, pr.us, pr.vs, pr.Kp,  \
-                                                  pr.Pp, c32, pr.t2, it.t, it.kb, it.pb, s0,    \
-                                                  s1, it.sp);                                   \
-  } while (0)
-    switch (pr.cfg) {
-      case 0: WINO_F8_TILE(2, 2, 4, 4); break;
-      case 1: WINO_F8_TILE(2, 2, 2, 4); break;
-      case 2: WINO_F8_TILE(2, 2, 2, 2); break;
-      case 3: WINO_F8_TILE(1, 4, 2, 2); break;
-      default: WINO_F8_TILE(1, 4, 1, 1); break;
-    }
-#undef WINO_F8_TILE
+    if (PLANES == 2 && pr.m == 4)
+      gemm_tile_f8<C::WK, C::WP, C::TK, C::TP, PLANES, true>(gsmem, U, V, pr.M, m32, pr.us, pr.vs,
+                                                             pr.Kp, pr.Pp, c32, pr.t2, it.t, it.kb,
+                                                             it.pb, s0, s1, it.sp);
+    else
+      gemm_tile_f8<C::WK, C::WP, C::TK, C::TP, PLANES, false>(gsmem, U, V, pr.M, m32, pr.us,
+                                                              pr.vs, pr.Kp, pr.Pp, c32, pr.t2, it.t,
+                                                              it.kb, it.pb, s0, s1, it.sp);
   }
 }
 
-size_t grouped_gemm_lds(int precision) {
-  if (precision == WINO_PREC_FP16) {
-    const int a = TileShape<2, 2, 4, 4>::lds_f16(), b = TileShape<4, 1, 4, 4>::lds_f16();
-    return (size_t)(a > b ? a : b);
-  }
-  return (size_t)TileShape<2, 2, 4, 4>::lds_f8(precision == WINO_PREC_FP8X2 ? 2 : 1);
+template <int CFG>
+static size_t cfg_lds(int precision) {
+  using C = kCfgT<CFG>;
+  using S = TileShape<C::WK, C::WP, C::TK, C::TP>;
+  if (precision == WINO_PREC_FP16) return (size_t)S::lds_f16();
+  return (size_t)S::lds_f8(precision == WINO_PREC_FP8X2 ? 2 : 1);
 }
 
-hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
-                               uint32_t grid, hipStream_t st) {
-  const size_t lds = grouped_gemm_lds(precision);
+template <int CFG>
+static hipError_t launch_cfg(int precision, const GroupHeader* hdr, const GProb* probs,
+                             uint32_t grid, hipStream_t st) {
+  const size_t lds = cfg_lds<CFG>(precision);
   if (precision == WINO_PREC_FP16) {
-    hipLaunchKernelGGL(k_gemm_grouped_f16, dim3(grid), dim3(256), lds, st, hdr, probs);
-  } else if (precision == WINO_PREC_FP8) {
-    hipLaunchKernelGGL(k_gemm_grouped_f8<1>, dim3(grid), dim3(256), lds, st, hdr, probs);
-  } else {
-    hipLaunchKernelGGL(k_gemm_grouped_f8<2>, dim3(grid), dim3(256), lds, st, hdr, probs);
+    hipLaunchKernelGGL(k_gemm_grouped_f16<CFG>, dim3(grid), dim3(256), lds, st, hdr, probs);
+  } else if constexpr (CFG < 5) {
+    if (precision == WINO_PREC_FP8)
+      hipLaunchKernelGGL((k_gemm_grouped_f8<CFG, 1>), dim3(grid), dim3(256), lds, st, hdr, probs);
+    else
+      hipLaunchKernelGGL((k_gemm_grouped_f8<CFG, 2>), dim3(grid), dim3(256), lds, st, hdr, probs);
   }
   return hipGetLastError();
 }
 
+hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
+                               uint32_t grid, uint32_t mask, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  if (mask & 1u) e = launch_cfg<0>(precision, hdr, probs, grid, st);
+  if (e == hipSuccess && (mask & 2u)) e = launch_cfg<1>(precision, hdr, probs, grid, st);
+  if (e == hipSuccess && (mask & 4u)) e = launch_cfg<2>(precision, hdr, probs, grid, st);
+  if (e == hipSuccess && (mask & 8u)) e = launch_cfg<3>(precision, hdr, probs, grid, st);
+  if (e == hipSuccess && (mask & 16u)) e = launch_cfg<4>(precision, hdr, probs, grid, st);
+  if (e == hipSuccess && (mask & 32u) && precision == WINO_PREC_FP16)
+    e = launch_cfg<5>(precision, hdr, probs, grid, st);
+  return e;
+}
+
+template <int CFG>
+static hipError_t prepare_cfg(int precision) {
+  const int lds = (int)cfg_lds<CFG>(precision);
+  if (precision == WINO_PREC_FP16)
+    return hipFuncSetAttribute((const void*)k_gemm_grouped_f16<CFG>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if constexpr (CFG < 5) {
+    if (precision == WINO_PREC_FP8)
+      return hipFuncSetAttribute((const void*)k_gemm_grouped_f8<CFG, 1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    return hipFuncSetAttribute((const void*)k_gemm_grouped_f8<CFG, 2>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  }
+  return hipSuccess;
+}
+
 hipError_t grouped_gemm_prepare(int precision) {
-  const int lds = (int)grouped_gemm_lds(precision);
-  hipError_t e = hipFuncSetAttribute((const void*)k_gemm_grouped_f16,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_gemm_grouped_f8<1>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_gemm_grouped_f8<2>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipError_t e = prepare_cfg<0>(precision);
+  if (e == hipSuccess) e = prepare_cfg<1>(precision);
+  if (e == hipSuccess) e = prepare_cfg<2>(precision);
+  if (e == hipSuccess) e = prepare_cfg<3>(precision);
+  if (e == hipSuccess) e = prepare_cfg<4>(precision);
+  if (e == hipSuccess) e = prepare_cfg<5>(precision);
+  return e;
 }
 
 }  // namespace wino

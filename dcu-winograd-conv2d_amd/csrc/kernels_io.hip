@@ -484,44 +484,58 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 // Fixed grids; virtual block v is mapped to its problem by a forward cursor
 // over the problems' first-block offsets.
 
-template <int PREC>
+template <int PREC, int M>
 __global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __restrict__ hdr,
                                                        const GProb* __restrict__ probs) {
-  extern __shared__ __attribute__((aligned(16))) half_t gxs[];
-  constexpr int CG = PREC == 0 ? 8 : 32;
-  const uint32_t total = hdr->in_blocks, np = hdr->nprob;
-  ProbCursor<&GProb::in_first> cur;
+  __shared__ __attribute__((aligned(16))) half_t gxs[kGroupInputLds / 2];
+  constexpr int CG = PREC == 0 ? 8 : 32, S = M == 4 ? 1 : 0;
+  const uint32_t total = hdr->in_blocks[S], np = hdr->nprob;
+  ProbCursor<InFirst<S>> cur;
   cur.init(probs, np);
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
     cur.seek(probs, np, v);
     const GProb& pr = probs[cur.p];
-    const uint32_t local = v - pr.in_first;
+    const uint32_t local = v - pr.in_first[S];
     const uint32_t bx = local % pr.in_gx, cg = local / pr.in_gx;
-    const bool evenw = (pr.ig.W & 1u) == 0;
-    if (pr.m == 2) {
-      if (evenw)
-        input_block<2, CG, PREC, true>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
-      else
-        input_block<2, CG, PREC, false>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
-    } else {
-      if (evenw)
-        input_block<4, CG, PREC, true>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
-      else
-        input_block<4, CG, PREC, false>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
-    }
-    __syncthreads();  // xs is restaged by the next virtual block
+    if ((pr.ig.W & 1u) == 0)
+      input_block<M, CG, PREC, true>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+    else
+      input_block<M, CG, PREC, false>(pr.x, pr.V, pr.ig, pr.amax, pr.vs, bx, cg, gxs, local == 0);
+    lds_barrier();  // xs is restaged by the next virtual block
   }
 }
 
+// LDS-staged output blocks of the problems without a reduction split.
+template <int M>
 __global__ __launch_bounds__(256) void k_output_grouped(const GroupHeader* __restrict__ hdr,
                                                         const GProb* __restrict__ probs) {
-  const uint32_t total = hdr->out_blocks, np = hdr->nprob;
-  ProbCursor<&GProb::out_first> cur;
+  __shared__ __attribute__((aligned(16))) half_t sm[OutTB<M>::LDS / 2];
+  constexpr int S = M == 4 ? 1 : 0;
+  const uint32_t total = hdr->out_blocks[S], np = hdr->nprob;
+  ProbCursor<OutFirst<S>> cur;
   cur.init(probs, np);
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
     cur.seek(probs, np, v);
     const GProb& pr = probs[cur.p];
-    const uint64_t idx = (uint64_t)(v - pr.out_first) * 256u + threadIdx.x;
+    const uint32_t local = v - pr.out_first[S];
+    const uint32_t k = local / pr.out_pblocks, pb = local - k * pr.out_pblocks;
+    output_block<M>(pr.M, pr.y, pr.K, pr.ig.TH, pr.ig.TW, pr.P, pr.Pp, pr.Kp, pr.OH, pr.OW, k,
+                    (uint64_t)pb * OutTB<M>::TB, sm);
+    lds_barrier();  // sm is restaged by the next virtual block
+  }
+}
+
+// Problems with a reduction split: the FP32 partials are summed per pair of
+// tiles (256 pairs per virtual block).
+__global__ __launch_bounds__(256) void k_output_grouped_split(const GroupHeader* __restrict__ hdr,
+                                                              const GProb* __restrict__ probs) {
+  const uint32_t total = hdr->out_blocks[2], np = hdr->nprob;
+  ProbCursor<OutFirst<2>> cur;
+  cur.init(probs, np);
+  for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
+    cur.seek(probs, np, v);
+    const GProb& pr = probs[cur.p];
+    const uint64_t idx = (uint64_t)(v - pr.out_first[2]) * 256u + threadIdx.x;
     if (pr.m == 2)
       output_pair<2>(pr.M, pr.M32, (int)pr.split, pr.y, pr.K, pr.ig.TH, pr.ig.TW, pr.P, pr.Pp,
                      pr.Kp, pr.OH, pr.OW, idx);
@@ -531,20 +545,28 @@ __global__ __launch_bounds__(256) void k_output_grouped(const GroupHeader* __res
   }
 }
 
+uint32_t out_tiles_per_block(int m) { return m == 4 ? OutTB<4>::TB : OutTB<2>::TB; }
+
+template <int PREC>
+static void launch_in(const GroupHeader* hdr, const GProb* probs, uint32_t grid, uint32_t mask,
+                      hipStream_t st) {
+  if (mask & 1u) hipLaunchKernelGGL((k_input_grouped<PREC, 2>), dim3(grid), dim3(256), 0, st, hdr, probs);
+  if (mask & 2u) hipLaunchKernelGGL((k_input_grouped<PREC, 4>), dim3(grid), dim3(256), 0, st, hdr, probs);
+}
+
 hipError_t launch_input_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
-                                uint32_t grid, hipStream_t st) {
-  if (precision == WINO_PREC_FP16)
-    hipLaunchKernelGGL(k_input_grouped<0>, dim3(grid), dim3(256), kGroupInputLds, st, hdr, probs);
-  else if (precision == WINO_PREC_FP8)
-    hipLaunchKernelGGL(k_input_grouped<1>, dim3(grid), dim3(256), kGroupInputLds, st, hdr, probs);
-  else
-    hipLaunchKernelGGL(k_input_grouped<2>, dim3(grid), dim3(256), kGroupInputLds, st, hdr, probs);
+                                uint32_t grid, uint32_t mask, hipStream_t st) {
+  if (precision == WINO_PREC_FP16) launch_in<0>(hdr, probs, grid, mask, st);
+  else if (precision == WINO_PREC_FP8) launch_in<1>(hdr, probs, grid, mask, st);
+  else launch_in<2>(hdr, probs, grid, mask, st);
   return hipGetLastError();
 }
 
 hipError_t launch_output_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_output_grouped, dim3(grid), dim3(256), 0, st, hdr, probs);
+                                 uint32_t mask, hipStream_t st) {
+  if (mask & 1u) hipLaunchKernelGGL(k_output_grouped<2>, dim3(grid), dim3(256), 0, st, hdr, probs);
+  if (mask & 2u) hipLaunchKernelGGL(k_output_grouped<4>, dim3(grid), dim3(256), 0, st, hdr, probs);
+  if (mask & 4u) hipLaunchKernelGGL(k_output_grouped_split, dim3(grid), dim3(256), 0, st, hdr, probs);
   return hipGetLastError();
 }
 

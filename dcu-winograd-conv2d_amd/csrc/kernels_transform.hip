@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_amax_grouped(const GroupHeader* __restr
                                                       const GProb* __restrict__ probs) {
   __shared__ float red[4];
   const uint32_t total = hdr->amax_blocks, np = hdr->nprob;
-  ProbCursor<&GProb::amax_first> cur;
+  ProbCursor<AmaxFirst> cur;
   cur.init(probs, np);
   for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
     cur.seek(probs, np, v);

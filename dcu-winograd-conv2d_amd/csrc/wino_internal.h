@@ -91,12 +91,18 @@ InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_
 // between batches, so one captured graph serves every batch.
 struct GroupHeader {
   uint32_t nprob;
-  uint32_t in_blocks;    // total virtual blocks of the input transform
-  uint32_t out_blocks;   // total virtual blocks of the output transform
-  uint32_t nitems;       // GEMM work items (tiles x reduction splits, all problems)
-  uint32_t amax_blocks;  // total virtual blocks of the fp8 input reduction
-  uint32_t pad[3];
+  uint32_t nitems;        // GEMM work items (tiles x reduction splits, all problems)
+  uint32_t amax_blocks;   // total virtual blocks of the fp8 input reduction
+  uint32_t pad;
+  uint32_t in_blocks[2];  // input-transform virtual blocks: [0] F(2x3), [1] F(4x3) problems
+  uint32_t out_blocks[3]; // output: [0] F(2x3), [1] F(4x3) LDS-staged blocks, [2] split pairs
+  uint32_t pad2;
+  // problems are ordered by GEMM tile configuration: the items of
+  // configuration c are [cfg_first[c], cfg_first[c] + cfg_items[c])
+  uint32_t cfg_first[8];
+  uint32_t cfg_items[8];
 };
+constexpr int kGemmCfgs = 6;  // entries of the GEMM tile table
 struct GProb {
   const _Float16* x;
   _Float16* y;
@@ -112,43 +118,64 @@ struct GProb {
   uint32_t K, Kp, Cp, m, t2, OH, OW;
   uint32_t cfg, split, nsteps;
   uint32_t in_gx;        // input-transform blocks along x (then ceil(C/CG) along y)
-  uint32_t in_first, in_count;
-  uint32_t out_first, out_count;
+  // virtual block ranges in each launch's space (count 0 where the problem
+  // belongs to another launch; firsts stay monotone across problems)
+  uint32_t in_first[2], in_count[2];
+  uint32_t out_first[3], out_count[3];
+  uint32_t out_pblocks;  // LDS-staged output blocks per output channel
   uint32_t amax_first, amax_count;
   // GEMM work items of this problem: [item_first, item_first + item_count),
   // item = ((t * kblocks + kb) * pblocks + pb) * split + sp
   uint32_t item_first, item_count;
   uint32_t kblocks, pblocks;
 };
-constexpr uint32_t kGroupInputLds = 48 * 1024;  // dynamic LDS of the grouped input kernel
+constexpr uint32_t kGroupInputLds = 24 * 1024;  // LDS budget of a grouped input block
 
 // Problem of virtual block / item `v` for a block that walks v upward
 // (v = blockIdx.x, blockIdx.x + gridDim.x, ...): p only moves forward, and
 // the next problem's first index is kept, so most steps load nothing.
-// FIRST selects the per-problem first-index field.
-template <uint32_t GProb::*FIRST>
+// F::first(problem) is the first index of the problem in this walk's space.
+template <typename F>
 struct ProbCursor {
   uint32_t p = 0, next = 0xffffffffu;
   __device__ inline void init(const GProb* probs, uint32_t np) {
     p = 0;
-    next = np > 1 ? probs[1].*FIRST : 0xffffffffu;
+    next = np > 1 ? F::first(probs[1]) : 0xffffffffu;
   }
   __device__ inline void seek(const GProb* probs, uint32_t np, uint32_t v) {
     while (v >= next) {
       ++p;
-      next = p + 1 < np ? probs[p + 1].*FIRST : 0xffffffffu;
+      next = p + 1 < np ? F::first(probs[p + 1]) : 0xffffffffu;
     }
   }
 };
+struct ItemFirst {
+  __device__ static uint32_t first(const GProb& p) { return p.item_first; }
+};
+struct AmaxFirst {
+  __device__ static uint32_t first(const GProb& p) { return p.amax_first; }
+};
+template <int S>
+struct InFirst {
+  __device__ static uint32_t first(const GProb& p) { return p.in_first[S]; }
+};
+template <int S>
+struct OutFirst {
+  __device__ static uint32_t first(const GProb& p) { return p.out_first[S]; }
+};
 
+// `mask` selects the launches (bit c: tile configuration c; input: bit 0
+// F(2x3), bit 1 F(4x3); output: those plus bit 2 for the split problems);
+// a launch with no work only reads the header, but a captured graph needs
+// every launch since later contents may use it.
 hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
-                               uint32_t grid, hipStream_t st);
+                               uint32_t grid, uint32_t mask, hipStream_t st);
 hipError_t grouped_gemm_prepare(int precision);
-size_t grouped_gemm_lds(int precision);
+uint32_t out_tiles_per_block(int m);  // tiles of one LDS-staged output block
 hipError_t launch_input_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
-                                uint32_t grid, hipStream_t st);
+                                uint32_t grid, uint32_t mask, hipStream_t st);
 hipError_t launch_output_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
-                                 hipStream_t st);
+                                 uint32_t mask, hipStream_t st);
 hipError_t launch_amax_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
                                hipStream_t st);
 

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(HERE, "libwinoconv.so")
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_ALLOC, ERR_LAUNCH, ERR_CAPACITY, ERR_BUSY, ERR_NOT_FOUND = range(8)
 # algorithms; 0..2 are the reference's winograd_select values (include/common.h:140-144)
 ALGO_AUTO, ALGO_F43, ALGO_F23, ALGO_F23_FUSED, ALGO_F43_FUSED = -1, 0, 1, 2, 3
+ALGO_LEAST_TRAFFIC = -2  # engine filters / batches: per problem, the smaller V and M
 ALGO_NAMES = {ALGO_F43: "f43", ALGO_F23: "f23", ALGO_F23_FUSED: "f23_fused", ALGO_F43_FUSED: "f43_fused"}
 # GEMM operand precisions
 PREC_FP16, PREC_FP8, PREC_FP8X2 = 0, 1, 2
@@ -60,7 +61,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("batches", C.c_uint64), ("max_inflight_batches", C.c_uint32),
                 ("max_batch_problems", C.c_uint32), ("pool_peak", C.c_size_t),
-                ("mean_batch_gpu_us", C.c_double), ("mean_queue_us", C.c_double)]
+                ("mean_batch_gpu_us", C.c_double), ("mean_queue_us", C.c_double),
+                ("max_batch_gpu_us", C.c_double), ("max_queue_us", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol include/*.h declares

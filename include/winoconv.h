@@ -35,6 +35,9 @@ const char* wino_status_string(int status);
 /* Values 0..2 equal the reference's winograd_select enum
  * (include/common.h:140-144). */
 enum {
+  WINO_ALGO_LEAST_TRAFFIC = -2, /* engine filters and batches only: per problem,
+                              the tile size whose transformed input and GEMM
+                              result are smaller (no reference counterpart) */
   WINO_ALGO_AUTO = -1,     /* the reference selector, src/conv2d.cpp:83-114 */
   WINO_ALGO_F43 = 0,       /* F(4x3) non-fused: 4 kernels                    */
   WINO_ALGO_F23 = 1,       /* F(2x3) non-fused: 4 kernels                    */
@@ -128,8 +131,11 @@ int wino_engine_destroy(wino_engine* engine);
 /* Tuning table written by the autotuner: one line per shape class,
  * "m K C P_min P_max cfg split" (whitespace separated, '#' comments). */
 int wino_engine_load_tuning(wino_engine* engine, const char* path);
-/* algo: WINO_ALGO_F23, WINO_ALGO_F43 or WINO_ALGO_AUTO (keep both, choose
- * per problem with the reference rule; its fused choice runs as F(2x3)). */
+/* algo: WINO_ALGO_F23 or WINO_ALGO_F43 (one tile size), WINO_ALGO_AUTO
+ * (keep both, choose per problem with the reference rule; its fused choice
+ * runs as F(2x3)) or WINO_ALGO_LEAST_TRAFFIC (keep both, choose per problem
+ * the tile size with fewer T*T*tiles, i.e. less V and M traffic: F(4x3)
+ * unless tile padding makes F(2x3) smaller). */
 int wino_engine_register_filter(wino_engine* engine, const void* weight, uint32_t k, uint32_t c,
                                 int algo, int* filter_id);
 /* Re-transforms a registered filter after its weights changed. */
@@ -191,6 +197,8 @@ typedef struct {
   size_t pool_peak;               /* most pool bytes in use at once       */
   double mean_batch_gpu_us;       /* device time per batch (HIP events)   */
   double mean_queue_us;           /* submit -> launch, per request        */
+  double max_batch_gpu_us;        /* longest batch on the device          */
+  double max_queue_us;            /* longest submit -> launch             */
 } wino_server_stats_t;
 int wino_server_stats(wino_server* server, wino_server_stats_t* stats);
 
