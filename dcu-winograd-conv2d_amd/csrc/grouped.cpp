@@ -277,6 +277,7 @@ struct wino_batch {
   // only when the batch changed or the launch stream changed
   char* dev = nullptr;
   char* host = nullptr;  // pinned staging of the same layout
+  void* host_dev = nullptr;  // its device-side address
   size_t probs_off = 0, bytes = 0;
   bool dirty = true;
   hipStream_t last_stream = nullptr;
@@ -480,6 +481,7 @@ int wino_batch_create(wino_engine* e, int max_problems, wino_batch** out) {
   b->bytes = a256(b->probs_off + sizeof(GProb) * (size_t)max_problems);
   if (hipMalloc((void**)&b->dev, b->bytes) != hipSuccess ||
       hipHostMalloc((void**)&b->host, b->bytes, hipHostMallocDefault) != hipSuccess ||
+      hipHostGetDevicePointer(&b->host_dev, b->host, 0) != hipSuccess ||
       hipEventCreateWithFlags(&b->copied, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
     if (b->dev) (void)hipFree(b->dev);
@@ -719,7 +721,7 @@ int wino_batch_launch(wino_batch* b, void* stream) {
     int rc = hipst(hipEventSynchronize(b->copied));
     if (rc) return rc;
     const size_t used = build_descriptors(b);
-    rc = hipst(hipMemcpyAsync(b->dev, b->host, used, hipMemcpyHostToDevice, st));
+    rc = hipst(launch_copy_words(b->host_dev, b->dev, (uint32_t)((used + 15) / 16), st));
     if (!rc) rc = hipst(hipEventRecord(b->copied, st));
     if (rc) return rc;
     b->dirty = false;
@@ -798,13 +800,18 @@ struct wino_server {
 
 static void server_loop(wino_server* s) {
   (void)hipSetDevice(s->device);
+  double last_activity = now_us();
   while (true) {
     bool progressed = false;
     uint32_t inflight = 0;
     // retire finished batches
     for (auto& sl : s->slots) {
       if (!sl.busy) continue;
-      if (hipEventQuery(sl.event) != hipSuccess) {
+      const double tq = now_us();
+      const hipError_t qe = hipEventQuery(sl.event);
+      const double dq = now_us() - tq;
+      if (dq > s->stats.max_query_call_us) s->stats.max_query_call_us = dq;
+      if (qe != hipSuccess) {
         (void)hipGetLastError();
         ++inflight;
         continue;
@@ -815,6 +822,8 @@ static void server_loop(wino_server* s) {
         (void)hipGetLastError();
         ms = 0.0f;
       }
+      const double de = now_us() - t;
+      if (de > s->stats.max_query_call_us) s->stats.max_query_call_us = de;
       {
         std::lock_guard<std::mutex> lk(s->mu);
         s->gpu_us_sum += 1e3 * ms;
@@ -880,9 +889,12 @@ static void server_loop(wino_server* s) {
         st = wino_batch_capture(sl.batch, sl.stream);
         sl.captured = st == WINO_OK;
       }
+      const double t_enq = now_us();
       if (!st) st = hipst(hipEventRecord(sl.start, sl.stream));
       if (!st) st = wino_batch_launch(sl.batch, sl.stream);
       if (!st) st = hipst(hipEventRecord(sl.event, sl.stream));
+      const double d_enq = now_us() - t_enq;
+      if (d_enq > s->stats.max_launch_call_us) s->stats.max_launch_call_us = d_enq;
       if (st) s->error = st;
       sl.reqs = taken;
       sl.busy = true;
@@ -905,13 +917,29 @@ static void server_loop(wino_server* s) {
       std::lock_guard<std::mutex> lk(s->mu);
       if (s->pending.empty()) break;
     }
+    // While batches are in flight, requests wait, or the last activity was
+    // recent, poll without sleeping: a timed sleep can cost the wake-up
+    // latency of an idle host core (milliseconds were measured), which
+    // would land on request latency.  Only a server idle for 2 ms blocks.
+    const double tnow = now_us();
+    if (progressed) last_activity = tnow;
     if (!progressed) {
-      std::unique_lock<std::mutex> lk(s->mu);
-      if (inflight)
-        s->cv.wait_for(lk, std::chrono::microseconds(20));
-      else
+      bool pending;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        pending = !s->pending.empty();
+      }
+      if (inflight || pending || tnow - last_activity < 2000.0) {
+        std::this_thread::yield();
+      } else {
+        std::unique_lock<std::mutex> lk(s->mu);
         s->cv.wait_for(lk, std::chrono::milliseconds(1),
                        [&] { return !s->pending.empty() || s->stop.load(); });
+      }
+      if (pending) {
+        const double d = now_us() - tnow;
+        if (d > s->stats.max_wait_pending_us) s->stats.max_wait_pending_us = d;
+      }
     }
   }
 }

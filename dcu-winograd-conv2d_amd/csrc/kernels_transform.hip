@@ -246,6 +246,22 @@ hipError_t launch_input_amax(const _Float16* x, uint64_t count, float* amax, hip
   return hipGetLastError();
 }
 
+// Copies n 16-byte words from host-mapped pinned memory to device memory (the
+// grouped launch's descriptors): an ordinary kernel on the batch's stream,
+// so the upload never goes through the runtime's copy path.
+__global__ __launch_bounds__(256) void k_copy_words(const uint4* __restrict__ src,
+                                                    uint4* __restrict__ dst, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+hipError_t launch_copy_words(const void* src, void* dst, uint32_t n16, hipStream_t st) {
+  if (n16 == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_words, dim3(div_up(n16, 256)), dim3(256), 0, st, (const uint4*)src,
+                     (uint4*)dst, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_quantize_e4m3(const float* x, uint8_t* q, uint64_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_quantize_e4m3, dim3(div_up((n + 3) / 4, 256)), dim3(256), 0, st, x, q, n);

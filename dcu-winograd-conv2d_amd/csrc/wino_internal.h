@@ -184,6 +184,7 @@ hipError_t launch_amax_grouped(const GroupHeader* hdr, const GProb* probs, uint3
 hipError_t launch_filter_transform(const Plan& pl, const _Float16* w, uint32_t K, uint32_t C,
                                    void* U, float* u_scale, hipStream_t st);
 hipError_t launch_quantize_e4m3(const float* x, uint8_t* q, uint64_t n, hipStream_t st);
+hipError_t launch_copy_words(const void* src, void* dst, uint32_t n16, hipStream_t st);
 hipError_t launch_input_amax(const _Float16* x, uint64_t count, float* amax, hipStream_t st);
 hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N, uint32_t C,
                                   uint32_t H, uint32_t W, void* V, const float* amax,

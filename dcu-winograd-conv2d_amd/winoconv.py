@@ -62,7 +62,9 @@ class ServerStats(C.Structure):
                 ("batches", C.c_uint64), ("max_inflight_batches", C.c_uint32),
                 ("max_batch_problems", C.c_uint32), ("pool_peak", C.c_size_t),
                 ("mean_batch_gpu_us", C.c_double), ("mean_queue_us", C.c_double),
-                ("max_batch_gpu_us", C.c_double), ("max_queue_us", C.c_double)]
+                ("max_batch_gpu_us", C.c_double), ("max_queue_us", C.c_double),
+                ("max_launch_call_us", C.c_double), ("max_query_call_us", C.c_double),
+                ("max_wait_pending_us", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol include/*.h declares

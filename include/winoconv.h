@@ -199,6 +199,10 @@ typedef struct {
   double mean_queue_us;           /* submit -> launch, per request        */
   double max_batch_gpu_us;        /* longest batch on the device          */
   double max_queue_us;            /* longest submit -> launch             */
+  double max_launch_call_us;      /* longest host time to enqueue a batch */
+  double max_query_call_us;       /* longest host time of a completion check */
+  double max_wait_pending_us;     /* longest idle wait of the server thread
+                                     while requests were pending           */
 } wino_server_stats_t;
 int wino_server_stats(wino_server* server, wino_server_stats_t* stats);
 
