@@ -484,8 +484,10 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 // Fixed grids; virtual block v is mapped to its problem by a forward cursor
 // over the problems' first-block offsets.
 
+// fp16: at most 128 VGPRs (4 waves per SIMD; measured faster than the
+// unconstrained 144-VGPR allocation despite a small spill)
 template <int PREC, int M>
-__global__ __launch_bounds__(256) void k_input_grouped(const GroupHeader* __restrict__ hdr,
+__global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_input_grouped(const GroupHeader* __restrict__ hdr,
                                                        const GProb* __restrict__ probs) {
   __shared__ __attribute__((aligned(16))) half_t gxs[kGroupInputLds / 2];
   constexpr int CG = PREC == 0 ? 8 : 32, S = M == 4 ? 1 : 0;
