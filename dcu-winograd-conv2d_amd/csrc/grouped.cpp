@@ -546,7 +546,7 @@ static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, voi
   g.M32 = en.bp.split > 1 ? (float*)(ws + en.bp.m32_off) : nullptr;
   g.vs = e->precision == WINO_PREC_FP16 ? nullptr : (float*)(ws + en.bp.sc_off);
   g.amax = e->precision == WINO_PREC_FP16 ? nullptr : (float*)(ws + en.bp.sc_off + 256);
-  g.ig = input_geom(pl, n, f.C, h, w, kGroupInputLds);
+  g.ig = input_geom(pl, n, f.C, h, w, true, kGroupInputLds);
   g.P = pl.P;
   g.Pp = pl.Pp;
   g.K = f.K;
@@ -564,7 +564,7 @@ static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, voi
   // padded tile columns (p >= P) only reach GEMM columns nobody reads.
   g.in_gx = n * g.ig.nwr * g.ig.nwc;
   const int ms = pl.m == 4 ? 1 : 0;
-  g.in_count[ms] = g.in_gx * g.ig.CGp;
+  g.in_count[ms] = g.in_gx * g.ig.NGB;
   const uint32_t tb = pl.m == 4 ? out_tiles_per_block(4) : out_tiles_per_block(2);
   g.out_pblocks = (uint32_t)((pl.P + tb - 1) / tb);
   if (en.bp.split > 1)

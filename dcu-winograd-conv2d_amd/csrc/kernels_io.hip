@@ -55,7 +55,7 @@ __host__ __device__ inline uint32_t in_lds_bytes(int m, int CG, uint32_t nth, ui
 template <int m, int CG, int PREC, bool EVENW>
 __device__ __attribute__((always_inline)) inline void input_block(const half_t* __restrict__ X, void* __restrict__ Vout,
                                    const InputGeom& g, const float* __restrict__ amax,
-                                   float* __restrict__ vscale, uint32_t bx, uint32_t cg,
+                                   float* __restrict__ vscale, uint32_t bx, uint32_t gb,
                                    half_t* xs, bool write_scale) {
   constexpr int T = m + 2, T2 = T * T;
   constexpr int CPT = PREC == 0 ? 2 : 4;  // channels per thread in the transform
@@ -70,6 +70,11 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
   const InLds L = in_lds_geom(m, nth, ntw);
   const int y0 = (int)(th0 * m) - 1, xa = (int)(tw0 * m) - 2;
   const int tid = threadIdx.x;
+  // channel groups cg0 .. cg0 + ng of CG channels each (several per block
+  // when the window has few tiles)
+  const uint32_t cg0 = gb * g.G;
+  const uint32_t ng = min(g.G, g.NG - cg0);
+  const int NCT = (int)ng * CG;  // channels staged
 
   // ---- stage
   {
@@ -78,8 +83,8 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
     const int dcol = tid % LWd, slot = tid / LWd;
     const int x = xa + 2 * dcol;
     const bool x0in = x >= 0 && x < (int)g.W, x1in = x + 1 >= 0 && x + 1 < (int)g.W;
-    const uint32_t c0 = cg * CG;
-    const uint32_t nch = c0 < g.C ? min((uint32_t)CG, g.C - c0) : 0u;
+    const uint32_t c0 = cg0 * CG;
+    const uint32_t nch = c0 < g.C ? min((uint32_t)NCT, g.C - c0) : 0u;
     const uint64_t plane = (uint64_t)g.H * g.W;
     // byte range of the group's real channels: < 2^32 (tensors hold < 2^31 elements)
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -100,7 +105,7 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
       int ch = slot / L.LH, lr = slot % L.LH;
       uint32_t goff = (uint32_t)ch * plane2 + (uint32_t)(y0 + lr) * W2 + 2u * (uint32_t)x;
       int loff = (ch * L.PS + lr * L.LWp) / 2 + dcol;
-      while (ch < CG) {
+      while (ch < NCT) {
         uint32_t val[SB];
         int dst[SB];
 #pragma unroll
@@ -116,7 +121,7 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
             const uint32_t c = __builtin_amdgcn_raw_buffer_load_b16(rs, in1 ? goff + 2u : 0u, 0, 0);
             val[u] = (in0 ? (a & 0xffffu) : 0u) | ((in1 ? (c & 0xffffu) : 0u) << 16);
           }
-          dst[u] = ch < CG ? loff : -1;
+          dst[u] = ch < NCT ? loff : -1;
           ch += dch;
           lr += dlr;
           goff += g_step;
@@ -151,17 +156,19 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
   constexpr int LANES_PER_TILE = CG / CPT;  // 4 (fp16) or 8 (fp8)
   const int cq = tid % LANES_PER_TILE;
   const int ntiles = (int)(nth * ntw);
+  const int items = ntiles * (int)ng;  // (channel group, tile) pairs, tiles fastest
   const uint64_t tplane = (uint64_t)g.CGp * g.Pp * CG;  // elements per tile position
-  for (int i = tid / LANES_PER_TILE; i < ntiles; i += 256 / LANES_PER_TILE) {
-    const int lth = i / (int)ntw, ltw = i - lth * (int)ntw;
+  for (int i = tid / LANES_PER_TILE; i < items; i += 256 / LANES_PER_TILE) {
+    const int grp = i / ntiles, ti = i - grp * ntiles;
+    const int lth = ti / (int)ntw, ltw = ti - lth * (int)ntw;
     const uint64_t p = ((uint64_t)n * g.TH + th0 + lth) * g.TW + tw0 + ltw;
-    const uint64_t off = ((uint64_t)cg * g.Pp + p) * CG + cq * CPT;
+    const uint64_t off = ((uint64_t)(cg0 + grp) * g.Pp + p) * CG + cq * CPT;
     float v[CPT][T2];
 #pragma unroll
     for (int cc = 0; cc < CPT; ++cc) {
       // LDS column ltw*m holds input column (tw0+ltw)*m - 2; the tile needs
       // the m+2 columns after it
-      const half_t* src = xs + (cq * CPT + cc) * L.PS + (lth * m) * L.LWp + ltw * m;
+      const half_t* src = xs + (grp * CG + cq * CPT + cc) * L.PS + (lth * m) * L.LWp + ltw * m;
       float d[T2];
 #pragma unroll
       for (int a = 0; a < T; ++a) {
@@ -411,7 +418,7 @@ void input_window(const Plan& pl, int CG, uint64_t budget, uint32_t& WR, uint32_
   WR = wr;
 }
 
-InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_t W,
+InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_t W, bool pad_groups,
                      uint64_t budget) {
   const int CG = pl.precision == WINO_PREC_FP16 ? 8 : 32;
   InputGeom g;
@@ -425,13 +432,24 @@ InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_
   g.nwr = div_up(g.TH, g.WR);
   g.nwc = div_up(g.TW, g.WC);
   g.CGp = pl.Cp / CG;
+  g.NG = pad_groups ? g.CGp : div_up(C, CG);
+  // A window of fewer tiles than half a transform pass (64 tiles of one
+  // group for fp16, 32 for fp8) takes several channel groups, as many as
+  // fill one pass and fit the LDS budget.
+  const uint32_t pass = CG == 8 ? 64u : 32u;
+  const uint32_t tiles = g.WR * g.WC;
+  uint32_t G = tiles < pass / 2 ? div_up(pass, tiles) : 1u;
+  if (G > g.NG) G = g.NG;
+  while (G > 1 && in_lds_bytes(pl.m, CG * (int)G, g.WR, g.WC) > budget) --G;
+  g.G = G;
+  g.NGB = div_up(g.NG, G);
   g.Pp = pl.Pp;
   return g;
 }
 
 static size_t input_lds(const Plan& pl, const InputGeom& g) {
   const int CG = pl.precision == WINO_PREC_FP16 ? 8 : 32;
-  return in_lds_bytes(pl.m, CG, g.WR, g.WC);
+  return in_lds_bytes(pl.m, CG * (int)g.G, g.WR, g.WC);
 }
 
 // ------------------------------------------------------ single problem
@@ -439,10 +457,9 @@ static size_t input_lds(const Plan& pl, const InputGeom& g) {
 hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N, uint32_t C,
                                   uint32_t H, uint32_t W, void* V, const float* amax,
                                   float* v_scale, hipStream_t st) {
-  const InputGeom g = input_geom(pl, N, C, H, W, 40u * 1024u);
-  const int CG = pl.precision == WINO_PREC_FP16 ? 8 : 32;
+  const InputGeom g = input_geom(pl, N, C, H, W, false, 40u * 1024u);
   const size_t lds = input_lds(pl, g);
-  dim3 grid((uint32_t)((uint64_t)N * g.nwr * g.nwc), div_up(C, CG));
+  dim3 grid((uint32_t)((uint64_t)N * g.nwr * g.nwc), g.NGB);
 #define WINO_IN(M, G, P)                                                                      \
   do {                                                                                        \
     if (W % 2 == 0)                                                                           \

@@ -77,11 +77,16 @@ struct InputGeom {
   uint32_t WR, WC;     // window in tiles
   uint32_t nwr, nwc;   // windows per image
   uint32_t CGp;        // channel groups in the padded layout (Cp / CG)
+  uint32_t NG;         // channel groups written: ceil(C / CG), or CGp when the
+                       // zero groups of the padding must be written too
+  uint32_t G;          // channel groups per block (small windows take several)
+  uint32_t NGB;        // blocks along the channels: ceil(NG / G)
   uint64_t Pp;
 };
 // Window of WR x WC tiles whose CG-channel LDS image fits `budget` bytes.
 void input_window(const Plan& pl, int CG, uint64_t budget, uint32_t& WR, uint32_t& WC);
-InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_t W,
+// pad_groups: also write the zero channel groups c >= C of the padded layout
+InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_t W, bool pad_groups,
                      uint64_t budget);
 
 // ------------------------------------------------- grouped execution
