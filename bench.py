@@ -136,6 +136,20 @@ def bench_conv(torch, wc, args):
                      "resident": resident, "op": op}
         log(f"conv {name}: {ms * 1e3:.1f} us, {res[name]['tflops']:.1f} eff TFLOP/s, "
             f"rel-L2 {err:.2e}")
+    if not args.only:
+        # side by side: the vendor library on the same data (torch's fp16
+        # conv2d runs MIOpen on ROCm; benchmark mode lets MIOpen pick its
+        # fastest solver).  Reported only, never the chosen path.
+        torch.backends.cudnn.benchmark = True
+        yt = torch.nn.functional.conv2d(dx, dw, padding=1)
+        ms = time_loop(torch, lambda: torch.nn.functional.conv2d(dx, dw, padding=1), args.steps,
+                       args.warmup)
+        err = rel_l2(host(torch, yt)[images], ref)
+        res["miopen"] = {"us": ms * 1e3, "tflops": F / (ms * 1e-3) / 1e12, "rel_l2": err,
+                         "resident": False, "op": None,
+                         "library": "MIOpen via torch.nn.functional.conv2d (fp16, benchmark mode)"}
+        log(f"conv miopen: {ms * 1e3:.1f} us, {res['miopen']['tflops']:.1f} eff TFLOP/s, "
+            f"rel-L2 {err:.2e}")
     return res, F
 
 
@@ -450,7 +464,8 @@ def main():
             roof["mfma_tflops"] = round(gf / 1e12, 2)
             roof["mfma_frac"] = round(gf / PEAK_MFMA["fp16"], 4)
     for v in conv.values():
-        v["op"].destroy()
+        if v["op"] is not None:
+            v["op"].destroy()
 
     fp8 = bench_fp8(torch, wc, args)
     grouped = bench_grouped(torch, wc, args, tuning)
@@ -476,8 +491,11 @@ def main():
             "chosen_rel_l2": float(f"{chosen['rel_l2']:.3e}"),
             "plain_rel_l2": float(f"{plain['rel_l2']:.3e}"),
             "paths": {k: {"us": round(v["us"], 2), "eff_tflops": round(v["tflops"], 2),
-                          "rel_l2": float(f"{v['rel_l2']:.3e}"), "resident_filter": v["resident"]}
+                          "rel_l2": float(f"{v['rel_l2']:.3e}"), "resident_filter": v["resident"],
+                          **({"library": v["library"]} if "library" in v else {})}
                       for k, v in conv.items()},
+            "chosen_over_miopen": (round(conv["miopen"]["us"] / chosen["us"], 3)
+                                   if "miopen" in conv else None),
             "fp8_workload": {"shape": "N=32 C=512 H=W=14 K=512 F(4x3)", "paths": fp8},
             "grouped": grouped,
             "serving": serving,
