@@ -30,22 +30,30 @@ int reference_select(const problem_t* pr) {
   return WINO_ALGO_F43;
 }
 
+bool winograd_ok(const problem_t* pr) {
+  return pr->r == 3 && pr->s == 3 && pr->u == 1 && pr->v == 1 && pr->p == 1 && pr->q == 1;
+}
+
 int validate(const problem_t* pr) {
   if (!pr) return WINO_ERR_INVALID;
-  if (pr->r != 3 || pr->s != 3 || pr->u != 1 || pr->v != 1 || pr->p != 1 || pr->q != 1)
+  if (pr->n == 0 || pr->c == 0 || pr->h == 0 || pr->w == 0 || pr->k == 0 || pr->r == 0 ||
+      pr->s == 0 || pr->u == 0 || pr->v == 0)
     return WINO_ERR_UNSUPPORTED;
-  if (pr->n == 0 || pr->c == 0 || pr->h == 0 || pr->w == 0 || pr->k == 0)
+  const uint64_t lim = 1ull << 31;
+  if (pr->p >= lim / 4 || pr->q >= lim / 4) return WINO_ERR_UNSUPPORTED;
+  // at least one output pixel (the reference formula would wrap around)
+  if ((uint64_t)pr->h + 2ull * pr->p < pr->r || (uint64_t)pr->w + 2ull * pr->q < pr->s)
     return WINO_ERR_UNSUPPORTED;
+  const uint64_t oh = ((uint64_t)pr->h + 2ull * pr->p - pr->r) / pr->u + 1;
+  const uint64_t ow = ((uint64_t)pr->w + 2ull * pr->q - pr->s) / pr->v + 1;
   // element counts of every tensor must fit the 32-bit grid decomposition
   const uint64_t in = (uint64_t)pr->n * pr->c * pr->h * pr->w;
-  const uint64_t out = (uint64_t)pr->n * pr->k * pr->h * pr->w;
-  const uint64_t lim = 1ull << 31;
-  if (in >= lim || out >= lim || (uint64_t)pr->k * pr->c * 9 >= lim) return WINO_ERR_UNSUPPORTED;
+  const uint64_t out = (uint64_t)pr->n * pr->k * oh * ow;
+  const uint64_t wt = (uint64_t)pr->k * pr->c * pr->r * pr->s;
+  if (in >= lim || out >= lim || wt >= lim || oh >= lim || ow >= lim) return WINO_ERR_UNSUPPORTED;
   return WINO_OK;
 }
 
-// GEMM tile choice: least padded work, then enough blocks to fill 256 CUs,
-// then the larger tile (more operand reuse).
 // Tile configuration by a roofline model of the batched GEMM: HBM bytes
 // (V is re-read once per row block of K, U once per XCD, M written once)
 // against MFMA time derated by the tile's operand reuse, divided by how well
@@ -79,9 +87,22 @@ int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax
 
 int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
   memset(pl, 0, sizeof(*pl));
-  if (algo == WINO_ALGO_AUTO) algo = reference_select(pr);
-  if (algo < WINO_ALGO_F43 || algo > WINO_ALGO_F43_FUSED) return WINO_ERR_INVALID;
+  if (algo == WINO_ALGO_AUTO) algo = winograd_ok(pr) ? reference_select(pr) : WINO_ALGO_DIRECT;
+  if (algo < WINO_ALGO_F43 || algo > WINO_ALGO_DIRECT) return WINO_ERR_INVALID;
   if (precision < WINO_PREC_FP16 || precision > WINO_PREC_FP8X2) return WINO_ERR_INVALID;
+  if (algo == WINO_ALGO_DIRECT) {
+    if (precision != WINO_PREC_FP16) return WINO_ERR_UNSUPPORTED;
+    pl->algo = algo;
+    pl->precision = precision;
+    pl->oh = (pr->h + 2 * pr->p - pr->r) / pr->u + 1;
+    pl->ow = (pr->w + 2 * pr->q - pr->s) / pr->v + 1;
+    pl->gemm_cfg = -1;
+    pl->bk = 64;  // output channels per block
+    pl->bp = 64;  // output pixels per block
+    pl->ok = 1;
+    return WINO_OK;  // no workspace
+  }
+  if (!winograd_ok(pr)) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F43_FUSED) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F23_FUSED && precision != WINO_PREC_FP16) return WINO_ERR_UNSUPPORTED;
   pl->algo = algo;
@@ -107,6 +128,7 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     // 256 zero bytes after U: the fused kernel's direct-to-LDS staging reads
     // padding from there
     pl->ws_bytes = align256(pl->u_bytes) + 256;
+    pl->ok = 1;
     return WINO_OK;
   }
   pl->gemm_cfg = choose_gemm_cfg(pr->k, pl->P, pl->t2, precision, 0);
@@ -142,6 +164,7 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     off += 256;
   }
   pl->ws_bytes = off;
+  pl->ok = 1;
   return WINO_OK;
 }
 
@@ -227,6 +250,10 @@ static int stage_output(Param* prm) {
 static int run(Param* prm) {
   const Plan& pl = prm->plan;
   int st = WINO_OK;
+  if (pl.algo == WINO_ALGO_DIRECT)
+    return hip_status(launch_direct(direct_geom(prm->n, prm->c, prm->h, prm->w, prm->k, prm->r,
+                                                prm->s, prm->u, prm->v, prm->p, prm->q),
+                                    prm->in, prm->weight, prm->out, prm->stream));
   if (!(prm->resident && prm->filter_ready)) {
     st = stage_filter(prm);
     if (st) return st;
@@ -326,7 +353,12 @@ int getkernelInfo(problem_t* problem, kernelInfo_t* kernelInfo, void* param) {
   if (kernelInfo) {
     memset(kernelInfo, 0, sizeof(*kernelInfo));
     const Plan& pl = prm->plan;
-    if (pl.algo == WINO_ALGO_F23_FUSED) {
+    if (pl.algo == WINO_ALGO_DIRECT) {
+      const uint64_t npix = (uint64_t)prm->n * pl.oh * pl.ow;
+      kernelInfo->blockx = (uint32_t)((npix + 63) / 64);
+      kernelInfo->blocky = (prm->k + 63) / 64;
+      kernelInfo->threadx = 256;
+    } else if (pl.algo == WINO_ALGO_F23_FUSED) {
       kernelInfo->blockx = (uint32_t)((pl.P + pl.bp - 1) / pl.bp) * (pl.Kp / pl.bk);
       kernelInfo->threadx = 512;
     } else {
@@ -358,7 +390,7 @@ void conv2d_fp16(const void* param_ptr) {
 int wino_run(const void* param_ptr) {
   Param* prm = as_param(param_ptr);
   if (!prm) return WINO_ERR_INVALID;
-  if (prm->plan.t2 == 0) return set_status(prm, WINO_ERR_UNSUPPORTED);
+  if (!prm->plan.ok) return set_status(prm, WINO_ERR_UNSUPPORTED);
   return set_status(prm, run(prm));
 }
 
@@ -370,14 +402,14 @@ int wino_param_status(const void* param) {
 int wino_param_set_algo(void* param, int algo) {
   Param* prm = as_param(param);
   if (!prm) return WINO_ERR_INVALID;
-  if (prm->plan.t2 == 0) return WINO_ERR_UNSUPPORTED;
+  if (!prm->plan.ok) return WINO_ERR_UNSUPPORTED;
   return set_status(prm, replan(prm, algo, prm->plan.precision));
 }
 
 int wino_param_set_precision(void* param, int precision) {
   Param* prm = as_param(param);
   if (!prm) return WINO_ERR_INVALID;
-  if (prm->plan.t2 == 0) return WINO_ERR_UNSUPPORTED;
+  if (!prm->plan.ok) return WINO_ERR_UNSUPPORTED;
   return set_status(prm, replan(prm, prm->plan.algo, precision));
 }
 
@@ -399,7 +431,8 @@ int wino_param_set_resident_filter(void* param, int on) {
 int wino_param_prepare_filter(void* param) {
   Param* prm = as_param(param);
   if (!prm) return WINO_ERR_INVALID;
-  if (prm->plan.t2 == 0) return WINO_ERR_UNSUPPORTED;
+  if (!prm->plan.ok) return WINO_ERR_UNSUPPORTED;
+  if (prm->plan.algo == WINO_ALGO_DIRECT) return set_status(prm, WINO_OK);  // uses W in place
   int st = stage_filter(prm);
   if (!st) prm->filter_ready = 1;
   return set_status(prm, st);
@@ -463,7 +496,7 @@ int wino_param_info(const void* param, wino_info_t* info) {
 static int stage_guard(const void* param, Param** out) {
   Param* prm = as_param(param);
   if (!prm) return WINO_ERR_INVALID;
-  if (prm->plan.t2 == 0 || !prm->ws) return WINO_ERR_UNSUPPORTED;
+  if (!prm->plan.ok || !prm->ws) return WINO_ERR_UNSUPPORTED;
   *out = prm;
   return WINO_OK;
 }

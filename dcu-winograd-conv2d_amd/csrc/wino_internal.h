@@ -31,7 +31,8 @@ struct Plan {
   uint64_t P;      // n * tiles_h * tiles_w
   uint32_t Kp, Cp;
   uint64_t Pp;
-  int gemm_cfg;    // index into the GEMM tile table
+  int gemm_cfg;    // index into the GEMM tile table (-1: no GEMM stage)
+  int ok;          // 1 once planned
   uint32_t bk, bp; // GEMM tile rows / columns
   int planes;      // 1, or 2 for the fp8 hi/lo split
   uint64_t u_off, v_off, m_off, us_off, vs_off, amax_off;
@@ -59,6 +60,7 @@ struct Param {
 // ------------------------------------------------------------ planning
 int reference_select(const problem_t* pr);
 int validate(const problem_t* pr);
+bool winograd_ok(const problem_t* pr);  // 3x3, stride 1, pad 1
 int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax);
 
 // ------------------------------------------------------------ GEMM table
@@ -198,6 +200,16 @@ hipError_t launch_gemm(const Plan& pl, const void* U, const void* V, _Float16* M
                        const float* u_scale, const float* v_scale, hipStream_t st);
 hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N, uint32_t K,
                                    _Float16* y, hipStream_t st);
+// Direct convolution (implicit GEMM) of any filter size, stride and padding.
+struct DirectGeom {
+  uint32_t N, C, H, W, K, R, S, U, V, P, Q, OH, OW;
+  uint32_t RS, RED;  // R*S, C*R*S (the reduction)
+  uint64_t NPIX;     // N*OH*OW (the GEMM columns)
+};
+DirectGeom direct_geom(uint32_t N, uint32_t C, uint32_t H, uint32_t W, uint32_t K, uint32_t R,
+                       uint32_t S, uint32_t U, uint32_t V, uint32_t P, uint32_t Q);
+hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16* w, _Float16* y,
+                         hipStream_t st);
 hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, uint32_t N,
                             uint32_t C, uint32_t H, uint32_t W, uint32_t K, _Float16* y,
                             hipStream_t st);

@@ -1,7 +1,7 @@
 // conv2d_demo.cpp -- command-line harness with the reference's semantics
 // (src/main.cpp:7-134): the same 11 positional arguments
 //
-//     conv2dfp16demo n c h w k r s u v p q [--algo auto|f43|f23|f23_fused]
+//     conv2dfp16demo n c h w k r s u v p q [--algo auto|f43|f23|f23_fused|direct]
 //                    [--precision fp16|fp8|fp8x2] [--iters N] [--no-verify]
 //
 // the same data (rand() % 255 / 255 for input and filter, src/main.cpp:35-43),
@@ -12,10 +12,11 @@
 // (include/verfiy.h:5-44), and an element fails when
 // |gpu - cpu| / gpu > 0.01 or gpu is NaN/Inf (src/main.cpp:118-129).
 //
-// Differences, each printed: a problem the operator does not compute is
-// reported and exits with status 2 (the reference computed garbage); the
-// verifier runs on all host cores (the reference used one); the workspace is
-// released with wino_param_destroy.
+// Differences: problems other than 3x3 stride 1 pad 1 run on the direct
+// (implicit-GEMM) fallback instead of producing garbage; a problem the
+// operator rejects is reported and exits with status 2; the verifier runs on
+// all host cores (the reference used one); the workspace is released with
+// wino_param_destroy.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -31,7 +32,7 @@
 
 static void usage() {
   fprintf(stderr,
-          "usage: conv2dfp16demo n c h w k r s u v p q [--algo auto|f43|f23|f23_fused]\n"
+          "usage: conv2dfp16demo n c h w k r s u v p q [--algo auto|f43|f23|f23_fused|direct]\n"
           "                      [--precision fp16|fp8|fp8x2] [--iters N] [--no-verify]\n");
 }
 
@@ -97,7 +98,8 @@ int main(int argc, char** argv) {
       algo = !strcmp(a, "f43") ? WINO_ALGO_F43
              : !strcmp(a, "f23") ? WINO_ALGO_F23
              : !strcmp(a, "f23_fused") ? WINO_ALGO_F23_FUSED
-                                       : WINO_ALGO_AUTO;
+             : !strcmp(a, "direct") ? WINO_ALGO_DIRECT
+                                    : WINO_ALGO_AUTO;
     } else if (!strcmp(argv[i], "--precision") && i + 1 < argc) {
       const char* a = argv[++i];
       precision = !strcmp(a, "fp8") ? WINO_PREC_FP8 : !strcmp(a, "fp8x2") ? WINO_PREC_FP8X2
@@ -147,9 +149,10 @@ int main(int argc, char** argv) {
   }
   wino_info_t info;
   wino_param_info(param, &info);
-  static const char* names[] = {"F(4x3) non-fused", "F(2x3) non-fused", "F(2x3) fused"};
+  static const char* names[] = {"F(4x3) non-fused", "F(2x3) non-fused", "F(2x3) fused",
+                                "F(4x3) fused", "direct (implicit GEMM)"};
   static const char* precs[] = {"fp16", "fp8", "fp8x2"};
-  printf("algorithm: %s, precision %s\n", info.algo >= 0 && info.algo < 3 ? names[info.algo] : "?",
+  printf("algorithm: %s, precision %s\n", info.algo >= 0 && info.algo < 5 ? names[info.algo] : "?",
          precs[info.precision]);
 
   conv2d_fp16(param);  // warm-up and result

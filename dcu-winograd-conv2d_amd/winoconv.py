@@ -24,9 +24,10 @@ LIB_PATH = os.path.join(HERE, "libwinoconv.so")
 # status codes (include/winoconv.h)
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_ALLOC, ERR_LAUNCH, ERR_CAPACITY, ERR_BUSY, ERR_NOT_FOUND = range(8)
 # algorithms; 0..2 are the reference's winograd_select values (include/common.h:140-144)
-ALGO_AUTO, ALGO_F43, ALGO_F23, ALGO_F23_FUSED, ALGO_F43_FUSED = -1, 0, 1, 2, 3
+ALGO_AUTO, ALGO_F43, ALGO_F23, ALGO_F23_FUSED, ALGO_F43_FUSED, ALGO_DIRECT = -1, 0, 1, 2, 3, 4
 ALGO_LEAST_TRAFFIC = -2  # engine filters / batches: per problem, the smaller V and M
-ALGO_NAMES = {ALGO_F43: "f43", ALGO_F23: "f23", ALGO_F23_FUSED: "f23_fused", ALGO_F43_FUSED: "f43_fused"}
+ALGO_NAMES = {ALGO_F43: "f43", ALGO_F23: "f23", ALGO_F23_FUSED: "f23_fused", ALGO_F43_FUSED: "f43_fused",
+              ALGO_DIRECT: "direct"}
 # GEMM operand precisions
 PREC_FP16, PREC_FP8, PREC_FP8X2 = 0, 1, 2
 PREC_NAMES = {PREC_FP16: "fp16", PREC_FP8: "fp8", PREC_FP8X2: "fp8x2"}
@@ -181,9 +182,10 @@ class Conv2d:
 
     def __init__(self, x: int, w: int, y: int, n: int, c: int, h: int, wd: int, k: int,
                  algo: int = ALGO_AUTO, precision: int = PREC_FP16, resident: bool = False,
-                 stream: int | None = None):
+                 stream: int | None = None, r: int = 3, s: int = 3, u: int = 1, v: int = 1,
+                 p: int = 1, q: int = 1):
         L = lib()
-        self.problem = make_problem(x, w, y, n, c, h, wd, k)
+        self.problem = make_problem(x, w, y, n, c, h, wd, k, r, s, u, v, p, q)
         size = C.c_int(0)
         check(L.getParamsize(C.byref(self.problem), C.byref(size)), "getParamsize")
         self._blob = C.create_string_buffer(size.value)

@@ -42,7 +42,12 @@ enum {
   WINO_ALGO_F43 = 0,       /* F(4x3) non-fused: 4 kernels                    */
   WINO_ALGO_F23 = 1,       /* F(2x3) non-fused: 4 kernels                    */
   WINO_ALGO_F23_FUSED = 2, /* F(2x3) fused: transform+GEMM+inverse in one    */
-  WINO_ALGO_F43_FUSED = 3  /* F(4x3) fused (no reference counterpart)        */
+  WINO_ALGO_F43_FUSED = 3, /* F(4x3) fused (no reference counterpart)        */
+  WINO_ALGO_DIRECT = 4     /* direct convolution (implicit GEMM) for any
+                              filter size, stride and padding; what AUTO picks
+                              for problems the Winograd paths do not compute
+                              (the reference returned wrong results for them,
+                              src/conv2d.cpp:24-114)                          */
 };
 
 /* GEMM-stage operand precision.  Transforms are FP32 and the GEMM always
@@ -57,9 +62,11 @@ enum {
  * without the reference's process-global tile state.  Returns the algo. */
 int wino_select_algo(const problem_t* problem);
 
-/* Checks that the problem is one this operator computes (3x3, stride 1,
- * pad 1, nonzero extents, sizes within 32-bit element counts).  The
- * reference accepted anything and computed garbage (src/conv2d.cpp:24-114). */
+/* Checks that the problem is one this operator computes: nonzero extents
+ * and strides, at least one output pixel, tensors within 32-bit element
+ * counts.  3x3 stride-1 pad-1 problems run on the Winograd paths, all others
+ * on WINO_ALGO_DIRECT.  The reference accepted anything and computed garbage
+ * for everything but 3x3 stride 1 pad 1 (src/conv2d.cpp:24-114). */
 int wino_validate(const problem_t* problem);
 
 /* -------------------------------------------------- parameter-blob control */

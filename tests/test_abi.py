@@ -40,8 +40,9 @@ def test_param_size_and_unsupported_problems(wc):
     size = C.c_int(0)
     assert L.getParamsize(C.byref(pr), C.byref(size)) == 0 and size.value > 0
     blob = C.create_string_buffer(size.value)
-    # non-3x3, strided and zero-extent problems are rejected before any HIP call
-    for bad in (dict(r=5, s=5), dict(u=2, v=2), dict(p=0, q=0), dict(n=0)):
+    # zero extents, zero strides and filters larger than the padded image are
+    # rejected before any HIP call
+    for bad in (dict(n=0), dict(k=0), dict(r=0), dict(u=0), dict(r=11, p=1), dict(s=9, q=0)):
         f = dict(n=1, c=8, h=8, wd=8, k=4)
         f.update({("wd" if k == "w" else k): v for k, v in bad.items()})
         pb = wc.make_problem(0, 0, 0, f["n"], f["c"], f["h"], f["wd"], f["k"],
@@ -53,6 +54,34 @@ def test_param_size_and_unsupported_problems(wc):
     junk = C.create_string_buffer(size.value)
     assert L.wino_run(junk) == wc.ERR_INVALID
     assert L.wino_param_destroy(junk) == wc.ERR_INVALID
+
+
+def test_general_problems_plan_the_direct_fallback(wc):
+    """Problems other than 3x3 stride 1 pad 1 plan WINO_ALGO_DIRECT, which
+    needs no workspace (so no HIP call happens here); Winograd algorithms
+    and fp8 are refused for them."""
+    L = wc.lib()
+    size = C.c_int(0)
+    L.getParamsize(None, C.byref(size))
+    for dims in (dict(r=5, s=5, p=2, q=2), dict(u=2, v=2), dict(p=0, q=0), dict(r=1, s=1, p=0, q=0),
+                 dict(r=5, s=3, u=2, v=1, p=2, q=0)):
+        blob = C.create_string_buffer(size.value)
+        pr = wc.make_problem(0, 0, 0, 2, 8, 9, 11, 4, **dims)
+        info = wc.KernelInfo()
+        assert L.getkernelInfo(C.byref(pr), C.byref(info), blob) == 0, dims
+        inf = wc.Info()
+        assert L.wino_param_info(blob, C.byref(inf)) == 0
+        r, s, u, v = dims.get("r", 3), dims.get("s", 3), dims.get("u", 1), dims.get("v", 1)
+        p, q = dims.get("p", 1), dims.get("q", 1)
+        oh, ow = (9 + 2 * p - r) // u + 1, (11 + 2 * q - s) // v + 1
+        assert inf.algo == wc.ALGO_DIRECT and inf.ws_bytes == 0
+        assert (inf.oh, inf.ow) == (oh, ow)
+        assert info.blockx == -(-2 * oh * ow // 64) and info.blocky == 1 and info.threadx == 256
+        for algo in (wc.ALGO_F43, wc.ALGO_F23, wc.ALGO_F23_FUSED):
+            assert L.wino_param_set_algo(blob, algo) == wc.ERR_UNSUPPORTED
+        assert L.wino_param_set_algo(blob, wc.ALGO_DIRECT) == 0
+        assert L.wino_param_set_precision(blob, wc.PREC_FP8) == wc.ERR_UNSUPPORTED
+        assert L.wino_param_destroy(blob) == 0
 
 
 def test_status_strings(wc):

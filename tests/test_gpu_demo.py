@@ -41,6 +41,11 @@ DEMO_CASES = [
     ((3, 40, 13, 17, 24, 3, 3, 1, 1, 1, 1), ("--algo", "f23"), "F(2x3) non-fused"),
     ((2, 32, 15, 15, 48, 3, 3, 1, 1, 1, 1), ("--algo", "f43"), "F(4x3) non-fused"),
     ((2, 32, 15, 15, 48, 3, 3, 1, 1, 1, 1), ("--algo", "f43", "--precision", "fp8x2"), "fp8x2"),
+    # not 3x3 / stride 1 / pad 1: the direct fallback
+    ((2, 8, 12, 12, 16, 5, 5, 1, 1, 2, 2), (), "direct"),
+    ((3, 16, 15, 17, 24, 3, 3, 2, 2, 1, 1), (), "direct"),
+    ((2, 32, 9, 9, 16, 1, 1, 1, 1, 0, 0), (), "direct"),
+    ((2, 16, 9, 9, 8, 3, 3, 1, 1, 1, 1), ("--algo", "direct"), "direct"),
 ]
 
 

@@ -1,0 +1,96 @@
+"""The general fallback (WINO_ALGO_DIRECT, implicit GEMM on MFMA) against the
+FP64 direct-convolution oracle, for filter sizes, strides and paddings the
+Winograd paths do not compute, through the reference entry points
+(getkernelInfo picks it for such problems).
+
+Tolerance: FP32 accumulation of fp16 products and one fp16 rounding of the
+result give rel-L2 vs FP64 of a few 1e-4; the bar is 1e-3.  The reference
+harness's own per-element check (|gpu - cpu| / gpu <= 0.01 against the
+verfiy.h restatement, positive data as src/main.cpp:35-43) must pass too."""
+import numpy as np
+import pytest
+
+DIRECT_CASES = [
+    # n, c, h, w, k, r, s, u, v, p, q
+    (2, 5, 9, 7, 3, 5, 3, 2, 1, 2, 0),
+    (1, 64, 14, 14, 32, 1, 1, 1, 1, 0, 0),
+    (3, 16, 15, 17, 40, 3, 3, 2, 2, 1, 1),
+    (2, 8, 8, 8, 70, 7, 7, 1, 1, 3, 3),
+    (2, 33, 10, 11, 65, 2, 2, 1, 1, 0, 0),
+    (1, 4, 5, 5, 2, 5, 5, 1, 1, 0, 0),          # one output pixel
+    (4, 128, 28, 28, 96, 1, 1, 2, 2, 0, 0),
+    (1, 3, 32, 32, 16, 3, 3, 1, 1, 1, 1),       # 3x3 s1 p1, forced direct
+    (2, 24, 13, 9, 12, 3, 5, 1, 3, 0, 2),
+]
+
+
+def _run(wc, gpu, x, wt, dims, algo=None):
+    n, c, h, w = x.shape
+    k, _, r, s = wt.shape
+    u, v, p, q = dims
+    oh, ow = (h + 2 * p - r) // u + 1, (w + 2 * q - s) // v + 1
+    dx, dw = gpu.to_dev(x), gpu.to_dev(wt)
+    dy = gpu.empty_f16((n, k, oh, ow))
+    op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k,
+                   algo=wc.ALGO_AUTO if algo is None else algo, r=r, s=s, u=u, v=v, p=p, q=q)
+    try:
+        op()
+        info = op.info()
+        y = gpu.to_host(dy)
+    finally:
+        op.destroy()
+    return y, info
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", DIRECT_CASES)
+def test_direct_matches_fp64(wc, gpu, oracle, case):
+    n, c, h, w, k, r, s, u, v, p, q = case
+    print("direct", case, flush=True)
+    rng = np.random.default_rng(sum(case))
+    x = rng.standard_normal((n, c, h, w)).astype(np.float16)
+    wt = (rng.standard_normal((k, c, r, s)) * np.sqrt(2.0 / (c * r * s))).astype(np.float16)
+    winograd = (r, s, u, v, p, q) == (3, 3, 1, 1, 1, 1)
+    y, info = _run(wc, gpu, x, wt, (u, v, p, q), algo=wc.ALGO_DIRECT if winograd else None)
+    assert info.algo == wc.ALGO_DIRECT
+    assert not np.isnan(y).any()
+    ref = oracle.conv2d_f64(x, wt, u, v, p, q)
+    assert y.shape == ref.shape
+    err = np.linalg.norm(y.astype(np.float64) - ref) / np.linalg.norm(ref)
+    assert err <= 1e-3, err
+    # every element within fp16 rounding of the FP32 sum (|y| 2^-11) plus the
+    # FP32 accumulation error bound over the reduction
+    red = c * r * s
+    bound = np.abs(ref) * 2.0 ** -10 + 2.0 ** -20 * red * np.abs(
+        oracle.conv2d_f64(np.abs(x.astype(np.float32)).astype(np.float16),
+                          np.abs(wt.astype(np.float32)).astype(np.float16), u, v, p, q)) + 1e-6
+    assert np.all(np.abs(y.astype(np.float64) - ref) <= bound)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", DIRECT_CASES[:4])
+def test_direct_passes_reference_harness_check(wc, gpu, oracle, case):
+    n, c, h, w, k, r, s, u, v, p, q = case
+    rng = np.random.default_rng(7)
+    x = (rng.integers(0, 255, (n, c, h, w)) / 255.0).astype(np.float16)
+    wt = (rng.integers(0, 255, (k, c, r, s)) / 255.0).astype(np.float16)
+    y, _ = _run(wc, gpu, x, wt, (u, v, p, q))
+    cpu = oracle.conv2d_verfiy(x, wt, u, v, p, q).astype(np.float32)
+    g = y.astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        bad = (np.abs(g - cpu) / g > 0.01) | np.isnan(g) | np.isinf(g)
+    assert not bad.any(), np.argwhere(bad)[:5]
+
+
+@pytest.mark.gpu
+def test_direct_equals_winograd_within_tolerance(wc, gpu, oracle):
+    """On a 3x3 s1 p1 problem the fallback and the F(4x3) path agree with
+    each other to their joint tolerance."""
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 32, 14, 14)).astype(np.float16)
+    wt = (rng.standard_normal((48, 32, 3, 3)) * 0.08).astype(np.float16)
+    yd, _ = _run(wc, gpu, x, wt, (1, 1, 1, 1), algo=wc.ALGO_DIRECT)
+    yw, _ = _run(wc, gpu, x, wt, (1, 1, 1, 1), algo=wc.ALGO_F43)
+    ref = oracle.conv2d_f64(x, wt)
+    assert np.linalg.norm(yd - ref) / np.linalg.norm(ref) <= 1e-3
+    assert np.linalg.norm(yd.astype(np.float64) - yw) / np.linalg.norm(ref) <= 1e-2

@@ -66,6 +66,23 @@ def test_fp64_direct_matches_torch(oracle):
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
+def test_fp64_direct_general_matches_torch(oracle):
+    """Unequal filter sides, strides and paddings (the direct fallback's
+    problems): a transposed index or a swapped stride fails this."""
+    rng = np.random.default_rng(11)
+    for (n, c, h, w, k, r, s, u, v, p, q) in [(2, 3, 9, 7, 4, 5, 3, 2, 1, 2, 0),
+                                              (1, 2, 6, 11, 3, 1, 4, 1, 3, 0, 2),
+                                              (1, 4, 8, 8, 2, 7, 7, 1, 1, 3, 3)]:
+        x = rng.standard_normal((n, c, h, w)).astype(np.float16)
+        wt = rng.standard_normal((k, c, r, s)).astype(np.float16)
+        got = oracle.conv2d_f64(x, wt, u, v, p, q)
+        ref = torch.nn.functional.conv2d(torch.from_numpy(x.astype(np.float64)),
+                                         torch.from_numpy(wt.astype(np.float64)),
+                                         stride=(u, v), padding=(p, q)).numpy()
+        assert got.shape == ref.shape
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
 def test_fp64_sample_matches_full(oracle):
     x, wt = rng_inputs(4, 2, 6, 7, 9, 4)
     full = oracle.conv2d_f64(x, wt).ravel()
