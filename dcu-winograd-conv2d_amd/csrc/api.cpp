@@ -97,8 +97,8 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     pl->oh = (pr->h + 2 * pr->p - pr->r) / pr->u + 1;
     pl->ow = (pr->w + 2 * pr->q - pr->s) / pr->v + 1;
     pl->gemm_cfg = -1;
-    pl->bk = 64;  // output channels per block
-    pl->bp = 64;  // output pixels per block
+    pl->bk = (uint32_t)direct_block_k(pr->k);  // output channels per block
+    pl->bp = 64;                               // output pixels per block
     pl->ok = 1;
     return WINO_OK;  // no workspace
   }
@@ -356,7 +356,7 @@ int getkernelInfo(problem_t* problem, kernelInfo_t* kernelInfo, void* param) {
     if (pl.algo == WINO_ALGO_DIRECT) {
       const uint64_t npix = (uint64_t)prm->n * pl.oh * pl.ow;
       kernelInfo->blockx = (uint32_t)((npix + 63) / 64);
-      kernelInfo->blocky = (prm->k + 63) / 64;
+      kernelInfo->blocky = (prm->k + pl.bk - 1) / pl.bk;
       kernelInfo->threadx = 256;
     } else if (pl.algo == WINO_ALGO_F23_FUSED) {
       kernelInfo->blockx = (uint32_t)((pl.P + pl.bp - 1) / pl.bp) * (pl.Kp / pl.bk);

@@ -208,6 +208,7 @@ struct DirectGeom {
 };
 DirectGeom direct_geom(uint32_t N, uint32_t C, uint32_t H, uint32_t W, uint32_t K, uint32_t R,
                        uint32_t S, uint32_t U, uint32_t V, uint32_t P, uint32_t Q);
+int direct_block_k(uint32_t K);  // output channels per direct block (64, 128 or 256)
 hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16* w, _Float16* y,
                          hipStream_t st);
 hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, uint32_t N,

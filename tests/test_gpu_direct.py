@@ -21,6 +21,8 @@ DIRECT_CASES = [
     (4, 128, 28, 28, 96, 1, 1, 2, 2, 0, 0),
     (1, 3, 32, 32, 16, 3, 3, 1, 1, 1, 1),       # 3x3 s1 p1, forced direct
     (2, 24, 13, 9, 12, 3, 5, 1, 3, 0, 2),
+    (2, 16, 10, 10, 200, 3, 3, 2, 2, 1, 1),     # 256-channel blocks
+    (1, 5, 7, 9, 300, 1, 3, 1, 1, 0, 1),        # 256-channel blocks, C*R*S odd
 ]
 
 
