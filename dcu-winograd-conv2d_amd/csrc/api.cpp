@@ -99,8 +99,13 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     pl->gemm_cfg = -1;
     pl->bk = (uint32_t)direct_block_k(pr->k);  // output channels per block
     pl->bp = 64;                               // output pixels per block
+    // workspace: the filter rearranged for the kernel, unless used in place
+    pl->u_bytes = direct_filter_bytes(direct_geom(pr->n, pr->c, pr->h, pr->w, pr->k, pr->r,
+                                                  pr->s, pr->u, pr->v, pr->p, pr->q));
+    pl->u_off = 0;
+    pl->ws_bytes = pl->u_bytes ? align256(pl->u_bytes) : 0;
     pl->ok = 1;
-    return WINO_OK;  // no workspace
+    return WINO_OK;
   }
   if (!winograd_ok(pr)) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F43_FUSED) return WINO_ERR_UNSUPPORTED;
@@ -211,8 +216,18 @@ static int alloc_workspace(Param* prm) {
 
 static char* wsp(const Param* prm, uint64_t off) { return (char*)prm->ws + off; }
 
+static DirectGeom param_direct_geom(const Param* prm) {
+  return direct_geom(prm->n, prm->c, prm->h, prm->w, prm->k, prm->r, prm->s, prm->u, prm->v,
+                     prm->p, prm->q);
+}
+
 static int stage_filter(Param* prm) {
   const Plan& pl = prm->plan;
+  if (pl.algo == WINO_ALGO_DIRECT) {
+    if (!pl.u_bytes) return WINO_OK;  // the kernel reads the filter in place
+    return hip_status(launch_direct_filter(param_direct_geom(prm), prm->weight,
+                                           (_Float16*)wsp(prm, pl.u_off), prm->stream));
+  }
   float* us = pl.precision != WINO_PREC_FP16 ? (float*)wsp(prm, pl.us_off) : nullptr;
   return hip_status(launch_filter_transform(pl, prm->weight, prm->k, prm->c, wsp(prm, pl.u_off),
                                             us, prm->stream));
@@ -250,14 +265,14 @@ static int stage_output(Param* prm) {
 static int run(Param* prm) {
   const Plan& pl = prm->plan;
   int st = WINO_OK;
-  if (pl.algo == WINO_ALGO_DIRECT)
-    return hip_status(launch_direct(direct_geom(prm->n, prm->c, prm->h, prm->w, prm->k, prm->r,
-                                                prm->s, prm->u, prm->v, prm->p, prm->q),
-                                    prm->in, prm->weight, prm->out, prm->stream));
   if (!(prm->resident && prm->filter_ready)) {
     st = stage_filter(prm);
     if (st) return st;
     prm->filter_ready = 1;
+  }
+  if (pl.algo == WINO_ALGO_DIRECT) {
+    const _Float16* wk = pl.u_bytes ? (const _Float16*)wsp(prm, pl.u_off) : prm->weight;
+    return hip_status(launch_direct(param_direct_geom(prm), prm->in, wk, prm->out, prm->stream));
   }
   if (pl.algo == WINO_ALGO_F23_FUSED) {
     return hip_status(launch_fused_f23(pl, prm->in, wsp(prm, pl.u_off), prm->n, prm->c, prm->h,
@@ -432,7 +447,6 @@ int wino_param_prepare_filter(void* param) {
   Param* prm = as_param(param);
   if (!prm) return WINO_ERR_INVALID;
   if (!prm->plan.ok) return WINO_ERR_UNSUPPORTED;
-  if (prm->plan.algo == WINO_ALGO_DIRECT) return set_status(prm, WINO_OK);  // uses W in place
   int st = stage_filter(prm);
   if (!st) prm->filter_ready = 1;
   return set_status(prm, st);
@@ -511,7 +525,7 @@ int wino_stage_input(const void* param) {
   Param* prm;
   int st = stage_guard(param, &prm);
   if (st) return st;
-  if (prm->plan.algo == WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
+  if (prm->plan.algo >= WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
   return set_status(prm, stage_input(prm));
 }
 
@@ -519,7 +533,7 @@ int wino_stage_gemm(const void* param) {
   Param* prm;
   int st = stage_guard(param, &prm);
   if (st) return st;
-  if (prm->plan.algo == WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
+  if (prm->plan.algo >= WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
   return set_status(prm, stage_gemm(prm));
 }
 
@@ -527,7 +541,7 @@ int wino_stage_output(const void* param) {
   Param* prm;
   int st = stage_guard(param, &prm);
   if (st) return st;
-  if (prm->plan.algo == WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
+  if (prm->plan.algo >= WINO_ALGO_F23_FUSED) return WINO_ERR_UNSUPPORTED;
   return set_status(prm, stage_output(prm));
 }
 

@@ -8,28 +8,35 @@
 // (include/verfiy.h:5-44).  Here
 //   Y[n][k][oy][ox] = sum over (c, kh, kw) of W[k][c][kh][kw] *
 //                     X[n][c][oy*u - p + kh][ox*v - q + kw]   (zero outside)
-// is a GEMM of the K x (C*R*S) filter matrix (KCRS rows, used in place) with
-// the (C*R*S) x (N*OH*OW) im2col matrix, which is never materialised.
+// is a GEMM of the K x (R*S*Cp) filter matrix with the (R*S*Cp) x (N*OH*OW)
+// im2col matrix (Cp = C rounded up to 8), which is never materialised.  The
+// reduction runs over (kh, kw, c) with the channel fastest, so 8 consecutive
+// reduction elements are 8 channels of one filter tap: one bounds check and
+// one offset per 8 elements, the 8 loads apart by a wave-uniform stride.
+// The filter is used in place when that order is already its layout (1x1
+// filters, C a multiple of 8); otherwise it is first rearranged into KRSC
+// with zero channels up to Cp (k_direct_filter, once per call, or once for a
+// resident filter).
 //
 // Block: 256 threads = 4 waves stacked along the output channels, each wave
 // TK x 4 MFMA tiles (v_mfma_f32_16x16x32_f16) = 16 TK channels x 64 output
 // pixels; the block covers 64 TK channels x 64 pixels.  Per 32-element
 // reduction step every wave loads its own filter rows straight into
-// registers (16-byte loads when C*R*S is a multiple of 8), while the block
-// gathers the 32 x 64 im2col slice into LDS (double-buffered, gathered two
-// steps ahead): thread (wave w, lane l) gathers reduction elements
-// 8w .. 8w+7 of the step for output pixel l.  The reduction index -> (c, kh,
-// kw) arithmetic is the same for all lanes of a wave, so it runs on the
-// scalar unit; a lane adds its pixel's offset and checks the image bounds.
-// FP32 accumulation, one fp16 rounding at the end.
+// registers (16-byte loads, two steps ahead), while the block gathers the
+// 32 x 64 im2col slice into LDS (double-buffered, gathered four steps
+// ahead): thread (wave w, lane l) gathers reduction elements 8w .. 8w+7 of
+// the step for output pixel l.  FP32 accumulation, one fp16 rounding at the
+// end.
+#include <algorithm>
+
 #include "wino_internal.h"
 #include "wino_math.h"
 
 namespace wino {
 
-template <int TK, bool VEC>
+template <int TK>
 __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X,
-                                                    const half_t* __restrict__ Wt,
+                                                    const half_t* __restrict__ Wk,
                                                     half_t* __restrict__ Y, DirectGeom g) {
   constexpr int BK = 64 * TK;
   __shared__ half8 Bs[2][4][64];  // [buffer][8-element group of the step][output pixel]
@@ -41,26 +48,31 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
   const uint32_t ohw = g.OH * g.OW, hw = g.H * g.W;
   const uint32_t nsteps = (g.RED + 31u) / 32u;
 
+  // All global reads are buffer loads: an offset past the buffer reads zero
+  // (and never faults), so masked rows and steps need no branch and every
+  // step issues the same loads.  Tensors hold < 2^31 elements, so byte
+  // offsets fit 32 bits; 0xFFFFFFF0 is past every buffer.
+  constexpr uint32_t kOut = 0xFFFFFFF0u;
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)Wk, 0, (int)(uint32_t)((uint64_t)g.K * g.RED * 2u), 0x00020000);
+  const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)X, 0, (int)(uint32_t)((uint64_t)g.N * g.C * hw * 2u), 0x00020000);
+
   // ---- filter rows of this wave: row k0 + 16 i + r16, elements 8 g4 .. 8 g4 + 7
-  const half_t* wrow[TK];
+  uint32_t wrow[TK];  // byte offset of the row's first element of this lane's group
   bool krow_ok[TK];
 #pragma unroll
   for (int i = 0; i < TK; ++i) {
     const uint32_t k = k0 + 16 * i + r16;
     krow_ok[i] = k < g.K;
-    wrow[i] = Wt + (uint64_t)(krow_ok[i] ? k : 0u) * g.RED + 8 * g4;
+    wrow[i] = (uint32_t)(((uint64_t)(krow_ok[i] ? k : 0u) * g.RED + 8u * g4) * 2u);
   }
   auto load_a = [&](uint32_t s, half8 (&a)[TK]) {
     const uint32_t j = s * 32u + 8u * g4;
 #pragma unroll
     for (int i = 0; i < TK; ++i) {
-      if constexpr (VEC) {
-        a[i] = (krow_ok[i] && j < g.RED) ? *(const half8*)(wrow[i] + s * 32u) : half8{};
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-          a[i][e] = (krow_ok[i] && j + e < g.RED) ? wrow[i][s * 32u + e] : (half_t)0;
-      }
+      const uint32_t o = (krow_ok[i] && j < g.RED) ? wrow[i] + s * 64u : kOut;
+      a[i] = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(wr, o, 0, 0));
     }
   };
 
@@ -71,27 +83,26 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
   const uint32_t pix = qb_ok ? (uint32_t)(qb - (uint64_t)nb * ohw) : 0u;
   const uint32_t oy = pix / g.OW, ox = pix - oy * g.OW;
   const int ybase = (int)(oy * g.U) - (int)g.P, xbase = (int)(ox * g.V) - (int)g.Q;
-  // element offset of (c, kh, kw) = pix_off + c*H*W + kh*W + kw (may start negative)
-  const int64_t pix_off = (int64_t)nb * g.C * hw + (int64_t)ybase * g.W + xbase;
+  // element offset of (c, kh, kw) = pix_off + c*H*W + kh*W + kw, in 32-bit
+  // modular arithmetic (pix_off may be "negative"; every in-image sum is
+  // below 2^31)
+  const uint32_t pix_off = (uint32_t)((int64_t)nb * g.C * hw + (int64_t)ybase * g.W + xbase);
+  const uint32_t plane2 = hw * 2u;
   auto gather = [&](uint32_t s, half8& b) {
-    // reduction elements j .. j+7 (wave-uniform): j -> (c, kh, kw)
+    // reduction elements j .. j+7 (wave-uniform): filter tap rs, channels c0 .. c0+7
     const uint32_t j = s * 32u + 8u * wave;
-    uint32_t c = j / g.RS;
-    const uint32_t rem = j - c * g.RS;
-    uint32_t kh = rem / g.S, kw = rem - kh * g.S;
+    const uint32_t rs = j / g.Cp, c0 = j - rs * g.Cp;
+    const uint32_t kh = rs / g.S, kw = rs - kh * g.S;
+    const int y = ybase + (int)kh, x = xbase + (int)kw;
+    const bool ok = qb_ok && rs < g.RS && (uint32_t)y < g.H && (uint32_t)x < g.W;
+    const uint32_t vo = ok ? (pix_off + c0 * hw + kh * g.W + kw) * 2u : kOut;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int y = ybase + (int)kh, x = xbase + (int)kw;
-      const bool ok = qb_ok && c < g.C && (uint32_t)y < g.H && (uint32_t)x < g.W;
-      const int64_t off = pix_off + (int64_t)c * hw + (int64_t)kh * g.W + kw;
-      b[e] = ok ? X[off] : (half_t)0;
-      if (++kw == g.S) {
-        kw = 0;
-        if (++kh == g.R) {
-          kh = 0;
-          ++c;
-        }
-      }
+      // channels past C (the padding up to Cp) read zero: their filter
+      // column is zero too, and an unrelated Inf would make 0 * Inf = NaN
+      const uint32_t so = c0 + e < g.C ? (uint32_t)e * plane2 : kOut;
+      b[e] = __builtin_bit_cast(half_t,
+                                (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(xr, vo, so, 0));
     }
   };
 
@@ -101,16 +112,24 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  half8 a[TK], b;
-  load_a(0, a);
-  gather(0, b);
-  Bs[0][wave][lane] = b;
-  if (nsteps > 1) gather(1, b);
+  // Register rings: filter rows of step x in aq[x % 2] (loaded two steps
+  // ahead), im2col slice of step x in bq[x % 4] (gathered four steps ahead,
+  // then written to LDS buffer x % 2 one step ahead).  The step count is
+  // padded to a multiple of 4; steps past the reduction load zeros (masked
+  // loads, no branches, so every path has the same loads in flight).
+  const uint32_t ns_pad = (nsteps + 3u) & ~3u;
+  half8 aq[2][TK], bq[4];
+  load_a(0, aq[0]);
+  load_a(1, aq[1]);
+  gather(0, bq[0]);
+  gather(1, bq[1]);
+  gather(2, bq[2]);
+  gather(3, bq[3]);
+  Bs[0][wave][lane] = bq[0];
+  gather(4, bq[0]);
   __syncthreads();
-  for (uint32_t s = 0; s < nsteps; ++s) {
+  auto step = [&](uint32_t s, half8 (&a)[TK], half8& bnext) {
     const int cur = s & 1;
-    half8 an[TK];
-    if (s + 1 < nsteps) load_a(s + 1, an);
     half8 fb[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) fb[j] = Bs[cur][g4][j * 16 + r16];
@@ -120,13 +139,16 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], fb[j], acc[i][j], 0, 0, 0);
-    if (s + 1 < nsteps) {
-      Bs[cur ^ 1][wave][lane] = b;  // step s + 1
-      if (s + 2 < nsteps) gather(s + 2, b);
-#pragma unroll
-      for (int i = 0; i < TK; ++i) a[i] = an[i];
-    }
+    Bs[cur ^ 1][wave][lane] = bnext;  // step s + 1 (unused after the last step)
+    gather(s + 5, bnext);
+    load_a(s + 2, a);
     __syncthreads();
+  };
+  for (uint32_t base = 0; base < ns_pad; base += 4) {
+    step(base, aq[0], bq[1]);
+    step(base + 1, aq[1], bq[2]);
+    step(base + 2, aq[0], bq[3]);
+    step(base + 3, aq[1], bq[0]);
   }
 
 #pragma unroll
@@ -143,6 +165,18 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
         const uint32_t k = k0 + i * 16 + 4 * g4 + e;
         if (k < g.K) yp[(uint64_t)k * ohw] = (half_t)acc[i][j][e];
       }
+  }
+}
+
+// W [K][C][R*S] -> Wk [K][R*S][Cp], zero channels c >= C.
+__global__ __launch_bounds__(256) void k_direct_filter(const half_t* __restrict__ W,
+                                                       half_t* __restrict__ Wk, DirectGeom g) {
+  const uint64_t total = (uint64_t)g.K * g.RED;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * 256u) {
+    const uint32_t k = (uint32_t)(i / g.RED), rem = (uint32_t)(i - (uint64_t)k * g.RED);
+    const uint32_t rs = rem / g.Cp, c = rem - rs * g.Cp;
+    Wk[i] = c < g.C ? W[((uint64_t)k * g.C + c) * g.RS + rs] : (half_t)0;
   }
 }
 
@@ -163,28 +197,37 @@ DirectGeom direct_geom(uint32_t N, uint32_t C, uint32_t H, uint32_t W, uint32_t 
   g.OH = (H + 2 * P - R) / U + 1;
   g.OW = (W + 2 * Q - S) / V + 1;
   g.RS = R * S;
-  g.RED = C * R * S;
+  g.Cp = (C + 7u) & ~7u;
+  g.RED = g.RS * g.Cp;
   g.NPIX = (uint64_t)N * g.OH * g.OW;
   return g;
+}
+
+bool direct_filter_in_place(const DirectGeom& g) { return g.RS == 1 && g.C % 8 == 0; }
+
+uint64_t direct_filter_bytes(const DirectGeom& g) {
+  return direct_filter_in_place(g) ? 0 : (uint64_t)g.K * g.RED * 2u;
 }
 
 // Channels per block: 256 (4 waves x 4 tiles) when K > 128, else 128 or 64.
 int direct_block_k(uint32_t K) { return K > 128 ? 256 : K > 64 ? 128 : 64; }
 
-hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16* w, _Float16* y,
+hipError_t launch_direct_filter(const DirectGeom& g, const _Float16* w, _Float16* wk,
+                                hipStream_t st) {
+  const uint64_t total = (uint64_t)g.K * g.RED;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_direct_filter, dim3(blocks), dim3(256), 0, st, (const half_t*)w,
+                     (half_t*)wk, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16* wk, _Float16* y,
                          hipStream_t st) {
   const int bk = direct_block_k(g.K);
   dim3 grid((uint32_t)((g.NPIX + 63) / 64), (g.K + bk - 1) / bk);
-  const bool vec = g.RED % 8 == 0;
-#define WINO_DIRECT(TK)                                                                        \
-  do {                                                                                         \
-    if (vec)                                                                                   \
-      hipLaunchKernelGGL((k_direct_f16<TK, true>), grid, dim3(256), 0, st, (const half_t*)x,   \
-                         (const half_t*)w, (half_t*)y, g);                                     \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_direct_f16<TK, false>), grid, dim3(256), 0, st, (const half_t*)x,  \
-                         (const half_t*)w, (half_t*)y, g);                                     \
-  } while (0)
+#define WINO_DIRECT(TK)                                                                   \
+  hipLaunchKernelGGL((k_direct_f16<TK>), grid, dim3(256), 0, st, (const half_t*)x,       \
+                     (const half_t*)wk, (half_t*)y, g)
   if (bk == 256)
     WINO_DIRECT(4);
   else if (bk == 128)

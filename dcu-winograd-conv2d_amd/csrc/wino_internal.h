@@ -203,13 +203,20 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 // Direct convolution (implicit GEMM) of any filter size, stride and padding.
 struct DirectGeom {
   uint32_t N, C, H, W, K, R, S, U, V, P, Q, OH, OW;
-  uint32_t RS, RED;  // R*S, C*R*S (the reduction)
-  uint64_t NPIX;     // N*OH*OW (the GEMM columns)
+  uint32_t RS, Cp, RED;  // R*S, C rounded up to 8, R*S*Cp (the reduction)
+  uint64_t NPIX;         // N*OH*OW (the GEMM columns)
 };
 DirectGeom direct_geom(uint32_t N, uint32_t C, uint32_t H, uint32_t W, uint32_t K, uint32_t R,
                        uint32_t S, uint32_t U, uint32_t V, uint32_t P, uint32_t Q);
 int direct_block_k(uint32_t K);  // output channels per direct block (64, 128 or 256)
-hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16* w, _Float16* y,
+// The direct kernel reads the filter as [K][R*S][Cp]; a KCRS filter already
+// is that when R*S == 1 and C % 8 == 0, else it is rearranged into a
+// workspace of direct_filter_bytes.
+bool direct_filter_in_place(const DirectGeom& g);
+uint64_t direct_filter_bytes(const DirectGeom& g);
+hipError_t launch_direct_filter(const DirectGeom& g, const _Float16* w, _Float16* wk,
+                                hipStream_t st);
+hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16* wk, _Float16* y,
                          hipStream_t st);
 hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, uint32_t N,
                             uint32_t C, uint32_t H, uint32_t W, uint32_t K, _Float16* y,

@@ -85,6 +85,63 @@ def test_direct_passes_reference_harness_check(wc, gpu, oracle, case):
 
 
 @pytest.mark.gpu
+def test_general_problems_plan_direct(wc, gpu):
+    """getkernelInfo plans WINO_ALGO_DIRECT for every problem outside 3x3
+    stride 1 pad 1, with the rearranged-filter workspace when the filter is
+    not already [K][R*S][C] with C a multiple of 8; Winograd algorithms and
+    fp8 are refused for such problems."""
+    import ctypes as C
+    L = wc.lib()
+    size = C.c_int(0)
+    L.getParamsize(None, C.byref(size))
+    for dims in (dict(r=5, s=5, p=2, q=2), dict(u=2, v=2), dict(p=0, q=0),
+                 dict(r=5, s=3, u=2, v=1, p=2, q=0)):
+        blob = C.create_string_buffer(size.value)
+        pr = wc.make_problem(0, 0, 0, 2, 5, 9, 11, 4, **dims)
+        info = wc.KernelInfo()
+        assert L.getkernelInfo(C.byref(pr), C.byref(info), blob) == 0, dims
+        inf = wc.Info()
+        assert L.wino_param_info(blob, C.byref(inf)) == 0
+        r, s, u, v = dims.get("r", 3), dims.get("s", 3), dims.get("u", 1), dims.get("v", 1)
+        p, q = dims.get("p", 1), dims.get("q", 1)
+        oh, ow = (9 + 2 * p - r) // u + 1, (11 + 2 * q - s) // v + 1
+        assert inf.algo == wc.ALGO_DIRECT and (inf.oh, inf.ow) == (oh, ow)
+        assert inf.u_bytes == 4 * r * s * 8 * 2  # K x R*S x Cp fp16, Cp = 8
+        assert info.blockx == -(-2 * oh * ow // 64) and info.blocky == 1
+        for algo in (wc.ALGO_F43, wc.ALGO_F23, wc.ALGO_F23_FUSED):
+            assert L.wino_param_set_algo(blob, algo) == wc.ERR_UNSUPPORTED
+        assert L.wino_param_set_algo(blob, wc.ALGO_DIRECT) == 0
+        assert L.wino_param_set_precision(blob, wc.PREC_FP8) == wc.ERR_UNSUPPORTED
+        assert L.wino_param_destroy(blob) == 0
+
+
+@pytest.mark.gpu
+def test_direct_resident_filter_and_rebinding(wc, gpu, oracle):
+    """A resident filter is rearranged once; rebinding a new filter of the
+    same shape makes the next call rearrange again."""
+    rng = np.random.default_rng(5)
+    n, c, h, w, k, r, s = 2, 6, 11, 10, 20, 5, 3
+    x = rng.standard_normal((n, c, h, w)).astype(np.float16)
+    w1 = (rng.standard_normal((k, c, r, s)) * 0.2).astype(np.float16)
+    w2 = (rng.standard_normal((k, c, r, s)) * 0.2).astype(np.float16)
+    oh, ow = (h + 2 - r) // 2 + 1, (w + 2 - s) // 1 + 1
+    dx, d1, d2 = gpu.to_dev(x), gpu.to_dev(w1), gpu.to_dev(w2)
+    dy = gpu.empty_f16((n, k, oh, ow))
+    op = wc.Conv2d(dx.data_ptr(), d1.data_ptr(), dy.data_ptr(), n, c, h, w, k, resident=True,
+                   r=r, s=s, u=2, v=1, p=1, q=1)
+    try:
+        for wt, dw in ((w1, d1), (w2, d2)):
+            op.set_tensors(dx.data_ptr(), dw.data_ptr(), dy.data_ptr())
+            op()
+            op()
+            y = gpu.to_host(dy).astype(np.float64)
+            ref = oracle.conv2d_f64(x, wt, 2, 1, 1, 1)
+            assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-3
+    finally:
+        op.destroy()
+
+
+@pytest.mark.gpu
 def test_direct_equals_winograd_within_tolerance(wc, gpu, oracle):
     """On a 3x3 s1 p1 problem the fallback and the F(4x3) path agree with
     each other to their joint tolerance."""
