@@ -190,6 +190,49 @@ def bench_fp8(torch, wc, args):
 
 
 # ------------------------------------------------------------------ grouped
+# ------------------------------------------------------------------ fallback
+FALLBACK_SHAPES = [  # (name, n, c, h, w, k, r, s, u, v, p, q)
+    ("1x1 s2 N=64 C=256 56x56 K=512", 64, 256, 56, 56, 512, 1, 1, 2, 2, 0, 0),
+    ("7x7 s2 p3 N=64 C=3 224x224 K=64", 64, 3, 224, 224, 64, 7, 7, 2, 2, 3, 3),
+]
+
+
+def bench_fallback(torch, wc, args):
+    """The direct (implicit-GEMM) fallback on problems the Winograd paths do
+    not compute, beside MIOpen (torch's fp16 conv2d) on the same data."""
+    out = {}
+    rng = np.random.default_rng(7)
+    for name, n, c, h, w, k, r, s, u, v, p, q in FALLBACK_SHAPES:
+        x = np.clip(rng.standard_normal((n, c, h, w)), -4, 4).astype(np.float16)
+        wt = (rng.standard_normal((k, c, r, s)) * np.sqrt(2.0 / (c * r * s))).astype(np.float16)
+        oh, ow = (h + 2 * p - r) // u + 1, (w + 2 * q - s) // v + 1
+        dx, dw = dev(torch, x), dev(torch, wt)
+        dy = torch.empty((n, k, oh, ow), dtype=torch.float16, device="cuda")
+        op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k,
+                       resident=True, r=r, s=s, u=u, v=v, p=p, q=q)
+        try:
+            op.prepare_filter()
+            ms = time_loop(torch, op, args.steps, args.warmup)
+            y = host(torch, dy)
+            assert op.info().algo == wc.ALGO_DIRECT
+        finally:
+            op.destroy()
+        ref = torch.nn.functional.conv2d(torch.from_numpy(x[:1].astype(np.float64)),
+                                         torch.from_numpy(wt.astype(np.float64)),
+                                         stride=(u, v), padding=(p, q)).numpy()
+        err = rel_l2(y[:1], ref)
+        torch.backends.cudnn.benchmark = True
+        f = lambda: torch.nn.functional.conv2d(dx, dw, stride=(u, v), padding=(p, q))
+        f()
+        ms_lib = time_loop(torch, f, args.steps, args.warmup)
+        F = 2.0 * n * k * oh * ow * c * r * s
+        out[name] = {"us": round(ms * 1e3, 2), "eff_tflops": round(F / (ms * 1e-3) / 1e12, 2),
+                     "rel_l2": float(f"{err:.3e}"), "miopen_us": round(ms_lib * 1e3, 2),
+                     "over_miopen": round(ms_lib / ms, 3)}
+        log(f"fallback {name}: {out[name]}")
+    return out
+
+
 def make_problems(rng, count, nmax):
     probs = []
     for _ in range(count):
@@ -383,6 +426,8 @@ def main():
     ap.add_argument("--only", default=None, help="run one conv path only (for profiling)")
     ap.add_argument("--serving-only", action="store_true", help="run the serving workload only")
     ap.add_argument("--grouped-only", action="store_true", help="run the grouped workload only")
+    ap.add_argument("--fallback-only", action="store_true",
+                    help="run the direct-fallback workloads only")
     ap.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
     ap.add_argument("--max-batch", type=int, default=16, help="serving: requests per batch")
     ap.add_argument("--no-graph", action="store_true",
@@ -414,6 +459,9 @@ def main():
         return
     if args.grouped_only:
         print(json.dumps(bench_grouped(torch, wc, args, tuning)), flush=True)
+        return
+    if args.fallback_only:
+        print(json.dumps(bench_fallback(torch, wc, args)), flush=True)
         return
     conv, F = bench_conv(torch, wc, args)
     if args.only:  # one path and its stages alone (the profiled command)
@@ -468,6 +516,7 @@ def main():
             v["op"].destroy()
 
     fp8 = bench_fp8(torch, wc, args)
+    fallback = bench_fallback(torch, wc, args)
     grouped = bench_grouped(torch, wc, args, tuning)
     serving = bench_serving(torch, wc, args, tuning)
 
@@ -497,6 +546,7 @@ def main():
             "chosen_over_miopen": (round(conv["miopen"]["us"] / chosen["us"], 3)
                                    if "miopen" in conv else None),
             "fp8_workload": {"shape": "N=32 C=512 H=W=14 K=512 F(4x3)", "paths": fp8},
+            "fallback": fallback,
             "grouped": grouped,
             "serving": serving,
         },
