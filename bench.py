@@ -260,6 +260,11 @@ def bench_grouped(torch, wc, args, tuning):
         for (n, c, h, w, k), fid, d in zip(probs, fids, data):
             batch.add(fid, d[2].data_ptr(), d[4].data_ptr(), n, h, w)
         algos = [batch.info(i).m for i in range(len(probs))]
+        # eager: only the kernels this mix needs; graph: one replay of every
+        # kernel the batch could need (empty ones exit at once)
+        with torch.cuda.stream(stream):
+            ms_eager = time_loop(torch, lambda: batch.launch(stream.cuda_stream), args.steps,
+                                 args.warmup)
         batch.capture(stream.cuda_stream)
         with torch.cuda.stream(stream):
             ms = time_loop(torch, lambda: batch.launch(stream.cuda_stream), args.steps, args.warmup)
@@ -287,6 +292,7 @@ def bench_grouped(torch, wc, args, tuning):
     out = {"problems": [list(p) for p in probs], "gflop": round(F / 1e9, 2),
            "algo_m": algos, "us": round(ms * 1e3, 2),
            "eff_tflops": round(F / (ms * 1e-3) / 1e12, 2),
+           "eager_us": round(ms_eager * 1e3, 2),
            "plain_us": round(ms_plain * 1e3, 2),
            "plain_eff_tflops": round(F / (ms_plain * 1e-3) / 1e12, 2),
            "grouped_over_plain": round(ms_plain / ms, 3),

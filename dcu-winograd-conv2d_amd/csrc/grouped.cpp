@@ -283,35 +283,97 @@ struct wino_batch {
   hipStream_t last_stream = nullptr;
   hipEvent_t copied = nullptr;  // the last descriptor copy has read `host`
   uint32_t grid_in = 0, grid_gemm = 0, grid_out = 0, grid_amax = 0;
+  // Graph mode (after wino_batch_capture): `exec` holds every kernel the
+  // batch could need, one graph node per launch-mask bit (`node`).  Before a
+  // replay the nodes of kernels without work in the current contents are
+  // disabled (and re-enabled when needed again), so a replay runs exactly
+  // the kernels an eager launch would, without capturing anything new.
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  hipGraphNode_t node[11] = {};
+  uint32_t enabled = 0;  // launch-mask bits whose nodes are enabled in `exec`
 };
 
 namespace wino {
 
-// Enqueues the batch's kernels.  `all`: every launch (graph capture, for any
-// later contents); otherwise only those with work in the current descriptors
-// (the host copy in `b->host` matches the device copy).
-static int batch_enqueue_kernels(wino_batch* b, hipStream_t st, bool all) {
+// Launch mask: bits 0-1 input kernels (F(2x3), F(4x3)), 2-7 GEMM tile
+// configurations, 8-10 output kernels (F(2x3), F(4x3), split pairs).
+constexpr uint32_t kAllKernels = 0x7ffu;
+
+// The kernels with work in the current descriptors (the host copy in
+// `b->host` matches the device copy).
+static uint32_t batch_launch_mask(const wino_batch* b) {
+  const GroupHeader* h = (const GroupHeader*)b->host;
+  uint32_t m = (h->in_blocks[0] ? 1u : 0u) | (h->in_blocks[1] ? 2u : 0u);
+  for (int c = 0; c < kGemmCfgs; ++c)
+    if (h->cfg_items[c]) m |= 4u << c;
+  m |= (h->out_blocks[0] ? 0x100u : 0u) | (h->out_blocks[1] ? 0x200u : 0u) |
+       (h->out_blocks[2] ? 0x400u : 0u);
+  return m;
+}
+
+// Enqueues the batch's kernels selected by `mask` (and the fp8 amax
+// reduction when `amax`).
+static int batch_enqueue_kernels(wino_batch* b, hipStream_t st, uint32_t mask, bool amax = true) {
   const GroupHeader* hdr = (const GroupHeader*)b->dev;
   const GProb* probs = (const GProb*)(b->dev + b->probs_off);
-  const GroupHeader* h = (const GroupHeader*)b->host;
-  uint32_t in_mask = 3u, gemm_mask = 0x3fu, out_mask = 7u;
-  if (!all) {
-    in_mask = (h->in_blocks[0] ? 1u : 0u) | (h->in_blocks[1] ? 2u : 0u);
-    out_mask = (h->out_blocks[0] ? 1u : 0u) | (h->out_blocks[1] ? 2u : 0u) |
-               (h->out_blocks[2] ? 4u : 0u);
-    gemm_mask = 0;
-    for (int c = 0; c < kGemmCfgs; ++c)
-      if (h->cfg_items[c]) gemm_mask |= 1u << c;
-  }
+  const uint32_t in_mask = mask & 3u, gemm_mask = (mask >> 2) & 0x3fu, out_mask = (mask >> 8) & 7u;
   const int prec = b->eng->precision;
   hipError_t e = hipSuccess;
-  if (prec != WINO_PREC_FP16) e = launch_amax_grouped(hdr, probs, b->grid_amax, st);
+  if (amax && prec != WINO_PREC_FP16) e = launch_amax_grouped(hdr, probs, b->grid_amax, st);
   if (e == hipSuccess) e = launch_input_grouped(prec, hdr, probs, b->grid_in, in_mask, st);
   if (e == hipSuccess) e = launch_gemm_grouped(prec, hdr, probs, b->grid_gemm, gemm_mask, st);
   if (e == hipSuccess) e = launch_output_grouped(hdr, probs, b->grid_out, out_mask, st);
   return hipst(e);
+}
+
+// Captures every kernel the batch could need into b->graph / b->exec, one
+// launch-mask bit at a time, recording the node each launch added.
+static int capture_all(wino_batch* b, hipStream_t st) {
+  int rc = hipst(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  if (rc) return rc;
+  const GroupHeader* hdr = (const GroupHeader*)b->dev;
+  const GProb* probs = (const GProb*)(b->dev + b->probs_off);
+  if (b->eng->precision != WINO_PREC_FP16)
+    rc = hipst(launch_amax_grouped(hdr, probs, b->grid_amax, st));  // always enabled
+  for (int bit = 0; bit < 11 && !rc; ++bit) {
+    rc = batch_enqueue_kernels(b, st, 1u << bit, false);
+    if (rc) break;
+    hipStreamCaptureStatus cs;
+    const hipGraphNode_t* deps = nullptr;
+    size_t ndeps = 0;
+    rc = hipst(hipStreamGetCaptureInfo_v2(st, &cs, nullptr, nullptr, &deps, &ndeps));
+    if (!rc && (cs != hipStreamCaptureStatusActive || ndeps != 1)) rc = WINO_ERR_LAUNCH;
+    if (!rc) b->node[bit] = deps[0];
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : hipst(e);
+  }
+  rc = hipst(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  if (rc) {
+    (void)hipGraphDestroy(g);
+    b->exec = nullptr;
+    return rc;
+  }
+  b->graph = g;
+  b->enabled = kAllKernels;
+  return WINO_OK;
+}
+
+// Graph-mode launch of the current contents (see wino_batch).
+static int batch_graph_launch(wino_batch* b, hipStream_t st) {
+  const uint32_t mask = batch_launch_mask(b);
+  const uint32_t diff = mask ^ b->enabled;
+  for (int bit = 0; bit < 11; ++bit) {
+    if (!(diff >> bit & 1u)) continue;
+    int rc = hipst(hipGraphNodeSetEnabled(b->exec, b->node[bit], (mask >> bit) & 1u));
+    if (rc) return rc;
+    b->enabled ^= 1u << bit;
+  }
+  return hipst(hipGraphLaunch(b->exec, st));
 }
 
 }  // namespace wino
@@ -727,8 +789,8 @@ int wino_batch_launch(wino_batch* b, void* stream) {
     b->dirty = false;
     b->last_stream = st;
   }
-  if (b->exec) return hipst(hipGraphLaunch(b->exec, st));
-  return batch_enqueue_kernels(b, st, false);
+  if (b->exec) return batch_graph_launch(b, st);
+  return batch_enqueue_kernels(b, st, batch_launch_mask(b));
 }
 
 int wino_batch_capture(wino_batch* b, void* stream) {
@@ -743,18 +805,7 @@ int wino_batch_capture(wino_batch* b, void* stream) {
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
-  int rc = hipst(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  if (rc) return rc;
-  rc = batch_enqueue_kernels(b, st, true);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(st, &g);
-  if (rc) {
-    if (g) (void)hipGraphDestroy(g);
-    return rc;
-  }
-  if (e != hipSuccess) return hipst(e);
-  b->graph = g;
-  return hipst(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return capture_all(b, st);
 }
 
 }  // extern "C"

@@ -210,36 +210,42 @@ def test_split_reduction_deterministic(wc, gpu, oracle):
 
 def test_graph_replay_new_contents(wc, gpu):
     """One capture, several replays with different batch contents and eager
-    work between them; every replay equals the eager launch."""
+    work between them; every replay equals the eager launch.  Each mix is
+    replayed three times (the all-kernels graph, then the capture of its own
+    kernel set on the second sighting, then that cached graph) and the
+    sequence of mixes runs twice, so cached sets are reused after others."""
     import torch
     S = Setup(wc, gpu, FP16)
     eager = wc.Batch(S.eng, 16)
     graph = wc.Batch(S.eng, 16)
     stream = torch.cuda.Stream()
     try:
-        sets = [[0, 2, 4], [1, 3, 5, 6, 7], [8], [0, 1, 2, 3, 4, 5, 6, 7, 8]]
+        sets = [[0, 2, 4], [1, 3, 5, 6, 7], [8], [0, 1, 2, 3, 4, 5, 6, 7, 8], [9], [3, 9]]
         captured = False
-        for r, js in enumerate(sets):
+        for r, js in enumerate(sets * 2):
             ye = [S.fresh_out(j) for j in js]
-            yg = [S.fresh_out(j) for j in js]
             eager.reset()
-            graph.reset()
-            for j, a, g in zip(js, ye, yg):
+            for j, a in zip(js, ye):
                 S.add(eager, j, a)
-                S.add(graph, j, g)
             eager.launch()
-            if not captured:
-                graph.capture(stream.cuda_stream)
-                captured = True
-            say(f"graph replay {r}: problems {js}")
-            graph.launch(stream.cuda_stream)
-            stream.synchronize()
-            # eager work between replays
-            junk = torch.randn(1 << 20, device="cuda:0").sum()
-            torch.cuda.synchronize()
-            del junk
-            for a, g in zip(ye, yg):
-                assert np.array_equal(gpu.to_host(a).view(np.uint16), gpu.to_host(g).view(np.uint16))
+            for rep in range(3):
+                yg = [S.fresh_out(j) for j in js]
+                graph.reset()
+                for j, g in zip(js, yg):
+                    S.add(graph, j, g)
+                if not captured:
+                    graph.capture(stream.cuda_stream)
+                    captured = True
+                say(f"graph replay {r}.{rep}: problems {js}")
+                graph.launch(stream.cuda_stream)
+                stream.synchronize()
+                # eager work between replays
+                junk = torch.randn(1 << 20, device="cuda:0").sum()
+                torch.cuda.synchronize()
+                del junk
+                for a, g in zip(ye, yg):
+                    assert np.array_equal(gpu.to_host(a).view(np.uint16),
+                                          gpu.to_host(g).view(np.uint16))
     finally:
         stream.synchronize()
         graph.destroy()
