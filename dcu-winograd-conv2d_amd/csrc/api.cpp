@@ -54,14 +54,23 @@ int validate(const problem_t* pr) {
   return WINO_OK;
 }
 
-// Tile configuration by a roofline model of the batched GEMM: HBM bytes
-// (V is re-read once per row block of K, U once per XCD, M written once)
-// against MFMA time derated by the tile's operand reuse, divided by how well
-// the tiles fill 256 CUs.  kmax > 0 limits the padded K rows a tile may
-// touch (the row count of an existing U layout).
-int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax) {
+// Tile configuration.  fp16 GEMMs with K >= 192 take the 256 x 64 tile
+// whenever it fits the padded K: it covers 256 output channels per block
+// with the register-fed A operand and was the fastest tile at every shape
+// and batch size the autotuner measured with K >= 256 (C = K = 256 at 14x14
+// and 28x28, C = K = 512 at 7x7 and 14x14; F(2x3) and F(4x3)).  Otherwise a
+// roofline model of the batched GEMM: HBM bytes (V re-read once per row
+// block of K, U once per XCD, M written once) against MFMA time derated by
+// the tile's operand reuse, divided by how well the tiles fill 256 CUs.
+// kmax > 0 limits the padded K rows a tile may touch (the row count of an
+// existing U layout).
+int choose_gemm_cfg(uint32_t K, uint32_t C, uint64_t P, int t2, int precision, uint32_t kmax) {
+  if (precision == WINO_PREC_FP16 && K >= 192 && gemm_cfg_ok(5, precision) &&
+      (kmax == 0 || round_up(K, (uint64_t)gemm_cfg(5).bk) <= kmax))
+    return 5;
   const double b_op = precision == WINO_PREC_FP16 ? 2.0 : (precision == WINO_PREC_FP8 ? 1.0 : 2.0);
   const double mfma = precision == WINO_PREC_FP16 ? 2.5e15 : (precision == WINO_PREC_FP8 ? 5.0e15 : 1.7e15);
+  const double cc = (double)C;
   int best = 0;
   double best_t = 1e300;
   for (int i = 0; i < gemm_cfg_count(); ++i) {
@@ -71,9 +80,10 @@ int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax
     if (kmax && round_up(K, c.bk) > kmax) continue;
     const double kp = (double)round_up(K, c.bk), pp = (double)round_up(P, c.bp);
     const double kb = kp / c.bk, pb = pp / c.bp;
-    const double bytes = t2 * (kb * pp * b_op + (pb < 8 ? pb : 8) * kp * b_op + kp * pp * 2.0);
+    const double bytes =
+        t2 * (kb * cc * pp * b_op + (pb < 8 ? pb : 8) * kp * cc * b_op + kp * pp * 2.0);
     const double reuse = (double)c.bk * c.bp / (c.bk + c.bp) / 64.0;  // 1 for 128 x 128
-    const double tm = 2.0 * t2 * kp * pp / (mfma * 0.5 * (reuse < 1.0 ? reuse : 1.0));
+    const double tm = 2.0 * t2 * kp * pp * cc / (mfma * 0.5 * (reuse < 1.0 ? reuse : 1.0));
     const double blocks = kb * pb * t2;
     const double fill = blocks >= 512.0 ? 1.0 : blocks / 512.0;
     const double t = (bytes / 6.0e12 > tm ? bytes / 6.0e12 : tm) / fill;
@@ -136,7 +146,7 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     pl->ok = 1;
     return WINO_OK;
   }
-  pl->gemm_cfg = choose_gemm_cfg(pr->k, pl->P, pl->t2, precision, 0);
+  pl->gemm_cfg = choose_gemm_cfg(pr->k, pr->c, pl->P, pl->t2, precision, 0);
   const GemmCfg gc = gemm_cfg(pl->gemm_cfg);
   pl->bk = gc.bk;
   pl->bp = gc.bp;

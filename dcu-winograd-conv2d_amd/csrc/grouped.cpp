@@ -172,7 +172,7 @@ static void lookup_tile(const wino_engine* e, uint32_t m, uint32_t K, uint32_t C
       *split = std::max(1, std::min<int>(te.split, (int)nsteps));
       return;
     }
-  *cfg = choose_gemm_cfg(K, P, t2, e->precision, (uint32_t)ru(K, 128));
+  *cfg = choose_gemm_cfg(K, C, P, t2, e->precision, (uint32_t)ru(K, 128));
   const GemmCfg gc = gemm_cfg(*cfg);
   const uint64_t tiles = (uint64_t)((K + gc.bk - 1) / gc.bk) * ((P + gc.bp - 1) / gc.bp) * t2;
   // a problem too small to fill the device on its own splits its reduction

@@ -61,7 +61,7 @@ struct Param {
 int reference_select(const problem_t* pr);
 int validate(const problem_t* pr);
 bool winograd_ok(const problem_t* pr);  // 3x3, stride 1, pad 1
-int choose_gemm_cfg(uint32_t K, uint64_t P, int t2, int precision, uint32_t kmax);
+int choose_gemm_cfg(uint32_t K, uint32_t C, uint64_t P, int t2, int precision, uint32_t kmax);
 
 // ------------------------------------------------------------ GEMM table
 struct GemmCfg {
