@@ -189,22 +189,20 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16_rows(
 #pragma unroll
     for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  // Register rings: A(s) in ra[(s - s0) % 2] (loaded two steps ahead),
-  // B(s) in rb[(s - s0) % 4] (loaded four steps ahead, written to LDS
-  // buffer (s - s0) % 2 one step ahead).  Loads past s1 re-read the last
-  // step (clamped) and are never used.
-  half8 ra[2][TK], rb[4];
+  half8 ra[2][TK], rb[3];
+  // prologue: A(s0), B(s0), B(s0+1), A(s0+1), B(s0+2) -- the per-step issue
+  // order below is A(s+2), B(s+3)
   load_a(s0, ra[0]);
   load_b(s0, rb[0]);
   load_b(s0 + 1, rb[1]);
   load_a(s0 + 1, ra[1]);
   load_b(s0 + 2, rb[2]);
-  load_b(s0 + 3, rb[3]);
   Bs[tid] = rb[0];
-  load_b(s0 + 4, rb[0]);
+  load_b(s0 + 3, rb[0]);
   lds_barrier();
 
-  auto step = [&](int s, half8 (&a)[TK], half8& bnext) {
+  auto step = [&](int s, half8 (&a)[TK], half8& bnext, half8& bfree) {
+    (void)bfree;
     const int cur = (s - s0) & 1;
     half8 b[TP];
 #pragma unroll
@@ -218,27 +216,14 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16_rows(
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[j], a[i], acc[i][j], 0, 0, 0);
     Bs[(cur ^ 1) * 4 * BP + tid] = bnext;  // B(s+1); unused after the last step
     load_a(s + 2, a);                       // a held A(s): refill with A(s+2)
-    load_b(s + 5, bnext);                   // bnext held B(s+1): refill with B(s+5)
+    load_b(s + 4, bnext);                   // bnext held B(s+1): refill with B(s+4)
     lds_barrier();
   };
-  // Whole groups of 4 steps run unconditionally, so every path through the
-  // loop has the same loads in flight and the compiler's wait counts keep
-  // the prefetch (a predicated step makes it wait for every load); the
-  // remaining < 4 steps follow.
-  int base = s0;
-  for (; base + 4 <= s1; base += 4) {
-    step(base, ra[0], rb[1]);
-    step(base + 1, ra[1], rb[2]);
-    step(base + 2, ra[0], rb[3]);
-    step(base + 3, ra[1], rb[0]);
-  }
-  for (; base < s1; ++base) {
-    switch ((base - s0) & 3) {
-      case 0: step(base, ra[0], rb[1]); break;
-      case 1: step(base, ra[1], rb[2]); break;
-      case 2: step(base, ra[0], rb[3]); break;
-      default: step(base, ra[1], rb[0]); break;
-    }
+  // slots: A(s) in ra[s % 2], B(s) in rb[s % 3] (relative to s0)
+  for (int base = s0; base < s1; base += 6) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (base + d < s1) step(base + d, ra[d % 2], rb[(d + 1) % 3], rb[d % 3]);
   }
 
   // acc[i][j][e]: channel k0 + 64 wave + 16 i + r16, tile p0 + 16 j + 4 g + e
