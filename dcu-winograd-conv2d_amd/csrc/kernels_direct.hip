@@ -9,10 +9,12 @@
 //   Y[n][k][oy][ox] = sum over (c, kh, kw) of W[k][c][kh][kw] *
 //                     X[n][c][oy*u - p + kh][ox*v - q + kw]   (zero outside)
 // is a GEMM of the K x (R*S*Cp) filter matrix with the (R*S*Cp) x (N*OH*OW)
-// im2col matrix (Cp = C rounded up to 8), which is never materialised.  The
-// reduction runs over (kh, kw, c) with the channel fastest, so 8 consecutive
-// reduction elements are 8 channels of one filter tap: one bounds check and
-// one offset per 8 elements, the 8 loads apart by a wave-uniform stride.
+// im2col matrix (Cp = C rounded up to 8, or 4 when C <= 4), which is never
+// materialised.  The reduction runs over (kh, kw, c) with the channel
+// fastest, so 8 consecutive reduction elements are 8 channels of one filter
+// tap (or 4 channels of two taps): one bounds check and one offset per tap,
+// the loads of a tap apart by a wave-uniform stride.  A filter row is
+// R*S*Cp elements rounded up to 8 (zero taps past R*S).
 // The filter is used in place when that order is already its layout (1x1
 // filters, C a multiple of 8); otherwise it is first rearranged into KRSC
 // with zero channels up to Cp (k_direct_filter, once per call, or once for a
@@ -34,7 +36,7 @@
 
 namespace wino {
 
-template <int TK>
+template <int TK, int CG>  // CG: channels per tap in a chunk of 8 (8, or 4 when Cp == 4)
 __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X,
                                                     const half_t* __restrict__ Wk,
                                                     half_t* __restrict__ Y, DirectGeom g) {
@@ -89,20 +91,25 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
   const uint32_t pix_off = (uint32_t)((int64_t)nb * g.C * hw + (int64_t)ybase * g.W + xbase);
   const uint32_t plane2 = hw * 2u;
   auto gather = [&](uint32_t s, half8& b) {
-    // reduction elements j .. j+7 (wave-uniform): filter tap rs, channels c0 .. c0+7
+    // reduction elements j .. j+7 (wave-uniform): 8 / CG filter taps of CG
+    // channels each, starting at tap rs0, channel c0
     const uint32_t j = s * 32u + 8u * wave;
-    const uint32_t rs = j / g.Cp, c0 = j - rs * g.Cp;
-    const uint32_t kh = rs / g.S, kw = rs - kh * g.S;
-    const int y = ybase + (int)kh, x = xbase + (int)kw;
-    const bool ok = qb_ok && rs < g.RS && (uint32_t)y < g.H && (uint32_t)x < g.W;
-    const uint32_t vo = ok ? (pix_off + c0 * hw + kh * g.W + kw) * 2u : kOut;
+    const uint32_t rs0 = j / g.Cp, c0 = j - rs0 * g.Cp;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      // channels past C (the padding up to Cp) read zero: their filter
-      // column is zero too, and an unrelated Inf would make 0 * Inf = NaN
-      const uint32_t so = c0 + e < g.C ? (uint32_t)e * plane2 : kOut;
-      b[e] = __builtin_bit_cast(half_t,
-                                (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(xr, vo, so, 0));
+    for (int h = 0; h < 8 / CG; ++h) {
+      const uint32_t rs = rs0 + h;
+      const uint32_t kh = rs / g.S, kw = rs - kh * g.S;
+      const int y = ybase + (int)kh, x = xbase + (int)kw;
+      const bool ok = qb_ok && rs < g.RS && (uint32_t)y < g.H && (uint32_t)x < g.W;
+      const uint32_t vo = ok ? (pix_off + c0 * hw + kh * g.W + kw) * 2u : kOut;
+#pragma unroll
+      for (int e = 0; e < CG; ++e) {
+        // channels past C (the padding up to Cp) read zero: their filter
+        // column is zero too, and an unrelated Inf would make 0 * Inf = NaN
+        const uint32_t so = c0 + e < g.C ? (uint32_t)e * plane2 : kOut;
+        b[h * CG + e] = __builtin_bit_cast(
+            half_t, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(xr, vo, so, 0));
+      }
     }
   };
 
@@ -168,7 +175,8 @@ __global__ __launch_bounds__(256) void k_direct_f16(const half_t* __restrict__ X
   }
 }
 
-// W [K][C][R*S] -> Wk [K][R*S][Cp], zero channels c >= C.
+// W [K][C][R*S] -> Wk [K][RED] with element rs * Cp + c, zero for c >= C and
+// for the taps rs >= R*S that round a row up to a multiple of 8.
 __global__ __launch_bounds__(256) void k_direct_filter(const half_t* __restrict__ W,
                                                        half_t* __restrict__ Wk, DirectGeom g) {
   const uint64_t total = (uint64_t)g.K * g.RED;
@@ -176,7 +184,7 @@ __global__ __launch_bounds__(256) void k_direct_filter(const half_t* __restrict_
        i += (uint64_t)gridDim.x * 256u) {
     const uint32_t k = (uint32_t)(i / g.RED), rem = (uint32_t)(i - (uint64_t)k * g.RED);
     const uint32_t rs = rem / g.Cp, c = rem - rs * g.Cp;
-    Wk[i] = c < g.C ? W[((uint64_t)k * g.C + c) * g.RS + rs] : (half_t)0;
+    Wk[i] = (c < g.C && rs < g.RS) ? W[((uint64_t)k * g.C + c) * g.RS + rs] : (half_t)0;
   }
 }
 
@@ -197,8 +205,8 @@ DirectGeom direct_geom(uint32_t N, uint32_t C, uint32_t H, uint32_t W, uint32_t 
   g.OH = (H + 2 * P - R) / U + 1;
   g.OW = (W + 2 * Q - S) / V + 1;
   g.RS = R * S;
-  g.Cp = (C + 7u) & ~7u;
-  g.RED = g.RS * g.Cp;
+  g.Cp = C <= 4 ? 4u : (C + 7u) & ~7u;
+  g.RED = (g.RS * g.Cp + 7u) & ~7u;
   g.NPIX = (uint64_t)N * g.OH * g.OW;
   return g;
 }
@@ -226,8 +234,14 @@ hipError_t launch_direct(const DirectGeom& g, const _Float16* x, const _Float16*
   const int bk = direct_block_k(g.K);
   dim3 grid((uint32_t)((g.NPIX + 63) / 64), (g.K + bk - 1) / bk);
 #define WINO_DIRECT(TK)                                                                   \
-  hipLaunchKernelGGL((k_direct_f16<TK>), grid, dim3(256), 0, st, (const half_t*)x,       \
-                     (const half_t*)wk, (half_t*)y, g)
+  do {                                                                                    \
+    if (g.Cp == 4)                                                                        \
+      hipLaunchKernelGGL((k_direct_f16<TK, 4>), grid, dim3(256), 0, st, (const half_t*)x, \
+                         (const half_t*)wk, (half_t*)y, g);                               \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_direct_f16<TK, 8>), grid, dim3(256), 0, st, (const half_t*)x, \
+                         (const half_t*)wk, (half_t*)y, g);                               \
+  } while (0)
   if (bk == 256)
     WINO_DIRECT(4);
   else if (bk == 128)

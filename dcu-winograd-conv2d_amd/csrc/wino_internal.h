@@ -203,7 +203,7 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 // Direct convolution (implicit GEMM) of any filter size, stride and padding.
 struct DirectGeom {
   uint32_t N, C, H, W, K, R, S, U, V, P, Q, OH, OW;
-  uint32_t RS, Cp, RED;  // R*S, C rounded up to 8, R*S*Cp (the reduction)
+  uint32_t RS, Cp, RED;  // R*S, C rounded up to 8 (4 if C <= 4), R*S*Cp rounded up to 8
   uint64_t NPIX;         // N*OH*OW (the GEMM columns)
 };
 DirectGeom direct_geom(uint32_t N, uint32_t C, uint32_t H, uint32_t W, uint32_t K, uint32_t R,
