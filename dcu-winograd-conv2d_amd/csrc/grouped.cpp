@@ -628,11 +628,12 @@ static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, voi
   const int ms = pl.m == 4 ? 1 : 0;
   g.in_count[ms] = g.in_gx * g.ig.NGB;
   const uint32_t tb = pl.m == 4 ? out_tiles_per_block(4) : out_tiles_per_block(2);
-  g.out_pblocks = (uint32_t)((pl.P + tb - 1) / tb);
+  g.out_kshift = out_kshift(pl.m, pl.P);
+  g.out_pblocks = (uint32_t)((pl.P + (tb >> g.out_kshift) - 1) / (tb >> g.out_kshift));
   if (en.bp.split > 1)
     g.out_count[2] = (uint32_t)(((uint64_t)f.K * ((pl.P + 1) / 2) + 255) / 256);
   else
-    g.out_count[ms] = f.K * g.out_pblocks;
+    g.out_count[ms] = ((f.K + (1u << g.out_kshift) - 1) >> g.out_kshift) * g.out_pblocks;
   const uint64_t nin = (uint64_t)n * f.C * h * w;
   g.amax_count = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, nin / (256 * 64)));
 }

@@ -338,11 +338,13 @@ __device__ __attribute__((always_inline)) inline void store_patch(half_t* __rest
   }
 }
 
-// LDS-staged output transform of one output channel k and OUT_TB<m> tiles
-// starting at p0: the block first copies M[t][k][p0 .. p0+TB) for all T*T
-// positions into LDS with 16-byte loads (a wave reads 1 KB runs of M, every
-// thread issues all its loads before the first LDS write), then each thread
-// transforms TB/256 tiles reading its T*T values from LDS.
+// LDS-staged output transform of 2^ks output channels k0 .. and TB / 2^ks
+// tiles starting at p0 (OutTB<m>::TB (channel, tile) pairs; several channels
+// per block when the problem has few tiles): the block first copies
+// M[t][k][p0 ..] for all T*T positions into LDS with 16-byte loads (a wave
+// reads contiguous runs of M, every thread issues all its loads before the
+// first LDS write), then each thread transforms TB/256 pairs reading its T*T
+// values from LDS.
 template <int m>
 struct OutTB {
   static constexpr int TB = m == 4 ? 256 : 512;  // tiles per block
@@ -354,22 +356,25 @@ template <int m>
 __device__ __attribute__((always_inline)) inline void output_block(const half_t* __restrict__ M, half_t* __restrict__ Y,
                                     uint32_t K, uint32_t TH, uint32_t TW, uint64_t P,
                                     uint64_t Pp, uint32_t Kp, uint32_t OH, uint32_t OW,
-                                    uint32_t k, uint64_t p0, half_t* sm) {
+                                    uint32_t k0, uint64_t p0, uint32_t ks, half_t* sm) {
   constexpr int T2 = OutTB<m>::T2, TB = OutTB<m>::TB;
   constexpr int CPR = TB / 8;                      // 16-byte chunks per position row
   constexpr int NCH = (T2 * CPR + 255) / 256;      // chunks per thread
   const int tid = threadIdx.x;
+  const int tbp = TB >> ks;                        // tiles per channel (>= 16)
   const uint64_t plane = (uint64_t)Kp * Pp;
-  const half_t* src = M + (uint64_t)k * Pp + p0;
+  const half_t* src = M + (uint64_t)k0 * Pp + p0;
   uint4 r[NCH];
 #pragma unroll
   for (int j = 0; j < NCH; ++j) {
     const int q = tid + j * 256;
     const int t = q / CPR, c = q - t * CPR;
-    // chunks past the padded tile count (last block) or past T2 rows are not
+    const int kk = c / (tbp / 8), cc = c - kk * (tbp / 8);
+    // chunks past the padded tile count, past K or past T2 rows are not
     // loaded; Pp is a multiple of 16, so a chunk is all in or all out
-    const bool ok = q < T2 * CPR && p0 + (uint64_t)c * 8 < Pp;
-    r[j] = ok ? *(const uint4*)(src + t * plane + c * 8) : uint4{0u, 0u, 0u, 0u};
+    const bool ok = q < T2 * CPR && p0 + (uint64_t)cc * 8 < Pp && k0 + kk < K;
+    r[j] = ok ? *(const uint4*)(src + t * plane + (uint64_t)kk * Pp + cc * 8)
+              : uint4{0u, 0u, 0u, 0u};
   }
 #pragma unroll
   for (int j = 0; j < NCH; ++j) {
@@ -380,8 +385,10 @@ __device__ __attribute__((always_inline)) inline void output_block(const half_t*
 #pragma unroll
   for (int h = 0; h < TB / 256; ++h) {
     const int i = tid + h * 256;
-    const uint64_t p = p0 + i;
-    if (p < P) {
+    const int kk = i >> (__builtin_ctz(TB) - ks), pi = i & (tbp - 1);
+    const uint64_t p = p0 + pi;
+    const uint32_t k = k0 + kk;
+    if (p < P && k < K) {
       float x[T2], y[m * m];
 #pragma unroll
       for (int t = 0; t < T2; ++t) x[t] = (float)sm[t * TB + i];
@@ -391,14 +398,26 @@ __device__ __attribute__((always_inline)) inline void output_block(const half_t*
   }
 }
 
+// log2 of the output channels per output block: a problem with fewer tiles
+// than a block's TB pairs takes several channels per block, keeping at least
+// 16 tiles per channel (32-byte runs of M).
+uint32_t out_kshift(int m, uint64_t P) {
+  const uint32_t tb = m == 4 ? OutTB<4>::TB : OutTB<2>::TB;
+  uint32_t ks = 0;
+  while ((tb >> (ks + 1)) >= 16 && (uint64_t)(tb >> (ks + 1)) >= P) ++ks;
+  return ks;
+}
+
 template <int m>
 __global__ __launch_bounds__(256) void k_output(const half_t* __restrict__ M,
                                                 half_t* __restrict__ Y, uint32_t K, uint32_t TH,
                                                 uint32_t TW, uint64_t P, uint64_t Pp, uint32_t Kp,
-                                                uint32_t OH, uint32_t OW, uint32_t pblocks) {
+                                                uint32_t OH, uint32_t OW, uint32_t pblocks,
+                                                uint32_t ks) {
   __shared__ __attribute__((aligned(16))) half_t sm[OutTB<m>::LDS / 2];
-  const uint32_t k = blockIdx.x / pblocks, pb = blockIdx.x - k * pblocks;
-  output_block<m>(M, Y, K, TH, TW, P, Pp, Kp, OH, OW, k, (uint64_t)pb * OutTB<m>::TB, sm);
+  const uint32_t kg = blockIdx.x / pblocks, pb = blockIdx.x - kg * pblocks;
+  output_block<m>(M, Y, K, TH, TW, P, Pp, Kp, OH, OW, kg << ks,
+                  (uint64_t)pb * (OutTB<m>::TB >> ks), ks, sm);
 }
 
 // --------------------------------------------------------------- geometry
@@ -486,14 +505,15 @@ hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N,
 hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N, uint32_t K,
                                    _Float16* y, hipStream_t st) {
   (void)N;
-  const uint32_t pblocks = div_up(pl.P, pl.m == 4 ? OutTB<4>::TB : OutTB<2>::TB);
-  dim3 grid(pblocks * K);
+  const uint32_t ks = out_kshift(pl.m, pl.P);
+  const uint32_t pblocks = div_up(pl.P, (pl.m == 4 ? OutTB<4>::TB : OutTB<2>::TB) >> ks);
+  dim3 grid(pblocks * div_up(K, 1u << ks));
   if (pl.m == 2)
     hipLaunchKernelGGL(k_output<2>, grid, dim3(256), 0, st, M, y, K, pl.tiles_h, pl.tiles_w, pl.P,
-                       pl.Pp, pl.Kp, pl.oh, pl.ow, pblocks);
+                       pl.Pp, pl.Kp, pl.oh, pl.ow, pblocks, ks);
   else
     hipLaunchKernelGGL(k_output<4>, grid, dim3(256), 0, st, M, y, K, pl.tiles_h, pl.tiles_w, pl.P,
-                       pl.Pp, pl.Kp, pl.oh, pl.ow, pblocks);
+                       pl.Pp, pl.Kp, pl.oh, pl.ow, pblocks, ks);
   return hipGetLastError();
 }
 
@@ -537,9 +557,10 @@ __global__ __launch_bounds__(256) void k_output_grouped(const GroupHeader* __res
     cur.seek(probs, np, v);
     const GProb& pr = probs[cur.p];
     const uint32_t local = v - pr.out_first[S];
-    const uint32_t k = local / pr.out_pblocks, pb = local - k * pr.out_pblocks;
-    output_block<M>(pr.M, pr.y, pr.K, pr.ig.TH, pr.ig.TW, pr.P, pr.Pp, pr.Kp, pr.OH, pr.OW, k,
-                    (uint64_t)pb * OutTB<M>::TB, sm);
+    const uint32_t kg = local / pr.out_pblocks, pb = local - kg * pr.out_pblocks;
+    output_block<M>(pr.M, pr.y, pr.K, pr.ig.TH, pr.ig.TW, pr.P, pr.Pp, pr.Kp, pr.OH, pr.OW,
+                    kg << pr.out_kshift, (uint64_t)pb * (OutTB<M>::TB >> pr.out_kshift),
+                    pr.out_kshift, sm);
     lds_barrier();  // sm is restaged by the next virtual block
   }
 }

@@ -129,7 +129,8 @@ struct GProb {
   // belongs to another launch; firsts stay monotone across problems)
   uint32_t in_first[2], in_count[2];
   uint32_t out_first[3], out_count[3];
-  uint32_t out_pblocks;  // LDS-staged output blocks per output channel
+  uint32_t out_pblocks;  // LDS-staged output blocks per group of output channels
+  uint32_t out_kshift;   // log2 of the output channels per output block
   uint32_t amax_first, amax_count;
   // GEMM work items of this problem: [item_first, item_first + item_count),
   // item = ((t * kblocks + kb) * pblocks + pb) * split + sp
@@ -178,7 +179,10 @@ struct OutFirst {
 hipError_t launch_gemm_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
                                uint32_t grid, uint32_t mask, hipStream_t st);
 hipError_t grouped_gemm_prepare(int precision);
-uint32_t out_tiles_per_block(int m);  // tiles of one LDS-staged output block
+uint32_t out_tiles_per_block(int m);
+// log2 of the output channels an output-transform block covers (several when
+// the problem has fewer tiles than a block's (channel, tile) pairs)
+uint32_t out_kshift(int m, uint64_t P);  // tiles of one LDS-staged output block
 hipError_t launch_input_grouped(int precision, const GroupHeader* hdr, const GProb* probs,
                                 uint32_t grid, uint32_t mask, hipStream_t st);
 hipError_t launch_output_grouped(const GroupHeader* hdr, const GProb* probs, uint32_t grid,
