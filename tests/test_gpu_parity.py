@@ -134,7 +134,7 @@ def _e2e(wc, gpu, oracle, x, wt, algo, prec, sample=None, label=""):
 
 E2E_SHAPES = [
     # (n, c, h, w, k, dist)
-    (1, 64, 56, 56, 64, "u11"),
+    (1, 64, 56, 56, 64, "u11"),      # N=1 C=K=64 56x56: the F(2x3) non-fused case
     (2, 3, 5, 5, 2, "u11"),          # tiny, C < 8
     (3, 17, 7, 9, 27, "clip4"),      # ragged C, K; non-square
     (2, 32, 14, 14, 64, "normal"),   # partial F(4x3) tiles (14 = 3*4 + 2)
@@ -245,24 +245,24 @@ def test_auto_selection_follows_reference(wc, gpu):
 
 
 @pytest.mark.parametrize("algo,prec", [(F43, FP16), (F23F, FP16), (F43, FP8X2), (F23, FP16)])
-def test_baseline_config2_sampled(wc, gpu, oracle, algo, prec):
+def test_n64_c256_28x28_k256_sampled(wc, gpu, oracle, algo, prec):
     """N=64 C=256 28x28 K=256 (the throughput configuration), 1% sample."""
     n, c, h, w, k = 64, 256, 28, 28, 256
     x, wt = rng_inputs(2026, n, c, h, w, k, dist="clip4")
     rng = np.random.default_rng(1)
     sample = rng.choice(n * k * h * w, size=n * k * h * w // 100, replace=False)
-    say(f"config2 algo={algo} prec={prec}")
-    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, algo, prec, sample=sample, label="cfg2")
+    say(f"N=64 C=256 28x28 K=256 algo={algo} prec={prec}")
+    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, algo, prec, sample=sample, label="n64c256")
     assert e64[1] <= TOL[(algo, prec)]
 
 
 @pytest.mark.parametrize("prec", [FP16, FP8X2, FP8])
-def test_baseline_config3_sampled(wc, gpu, oracle, prec):
+def test_n32_c512_14x14_k512_sampled(wc, gpu, oracle, prec):
     """F(4x3), N=32 C=512 14x14 K=512, FP16 vs FP8 (1% sample)."""
     n, c, h, w, k = 32, 512, 14, 14, 512
     x, wt = rng_inputs(2027, n, c, h, w, k, dist="normal")
     rng = np.random.default_rng(2)
     sample = rng.choice(n * k * h * w, size=n * k * h * w // 100, replace=False)
-    say(f"config3 prec={prec}")
-    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, F43, prec, sample=sample, label="cfg3")
+    say(f"N=32 C=512 14x14 K=512 prec={prec}")
+    _, e64, _, _ = _e2e(wc, gpu, oracle, x, wt, F43, prec, sample=sample, label="n32c512")
     assert e64[1] <= TOL[(F43, prec)]
