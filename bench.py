@@ -190,6 +190,64 @@ def bench_fp8(torch, wc, args):
 
 
 # ------------------------------------------------------------------ grouped
+# ------------------------------------------------------------------ reference cases
+# The six shapes of the reference's own benchmark script (final.sh:2-7, run
+# through its harness src/main.cpp: 100 calls, filter transformed on every
+# call, algorithm from its selector) with the latencies it published for a
+# Hygon DCU (performance.png, read off the chart to about +-10 us; other
+# hardware, so a reference point rather than a ratio).
+REFERENCE_CASES = [  # (n, c, h, w, k, published Winograd us, published MIOpen us)
+    (16, 128, 64, 64, 27, 210, 130),
+    (16, 256, 32, 32, 256, 370, 355),
+    (16, 64, 128, 128, 64, 725, 420),
+    (2, 1920, 32, 32, 640, 655, 780),
+    (2, 640, 64, 64, 640, 595, 995),
+    (2, 320, 64, 64, 4, 135, 80),
+]
+
+
+def bench_reference_cases(torch, wc, args):
+    """The reference's benchmark shapes through the reference entry points:
+    the selector's algorithm and the filter transformed on every call (as
+    the reference harness runs them), the fastest of this library's
+    algorithms under the same conditions, and MIOpen on the same data."""
+    out = []
+    rng = np.random.default_rng(11)
+    algos = [("f43", wc.ALGO_F43), ("f23", wc.ALGO_F23), ("f23_fused", wc.ALGO_F23_FUSED)]
+    for (n, c, h, w, k, ref_us, ref_miopen_us) in REFERENCE_CASES:
+        x = (rng.integers(0, 255, (n, c, h, w)) / 255.0).astype(np.float16)  # main.cpp data
+        wt = (rng.integers(0, 255, (k, c, 3, 3)) / 255.0).astype(np.float16)
+        dx, dw = dev(torch, x), dev(torch, wt)
+        dy = torch.empty((n, k, h, w), dtype=torch.float16, device="cuda")
+        ref = ref_f64(torch, x, wt, [0])
+        F = flops(n, c, h, w, k)
+        times = {}
+        for name, algo in [("selector", wc.ALGO_AUTO)] + algos:
+            op = wc.Conv2d(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k, algo=algo)
+            try:
+                sel = wc.ALGO_NAMES[op.info().algo]
+                ms = time_loop(torch, op, args.steps, args.warmup)
+                err = rel_l2(host(torch, dy)[:1], ref)
+            finally:
+                op.destroy()
+            times[name] = (ms * 1e3, err, sel)
+        best = min((t for t in times if t != "selector"), key=lambda t: times[t][0])
+        torch.backends.cudnn.benchmark = True
+        f = lambda: torch.nn.functional.conv2d(dx, dw, padding=1)
+        f()
+        ms_lib = time_loop(torch, f, args.steps, args.warmup)
+        e = {"shape": [n, c, h, w, k],
+             "selector_algo": times["selector"][2], "selector_us": round(times["selector"][0], 2),
+             "selector_eff_tflops": round(F / (times["selector"][0] * 1e-6) / 1e12, 2),
+             "fastest_algo": best, "fastest_us": round(times[best][0], 2),
+             "rel_l2": float(f"{times['selector'][1]:.3e}"),
+             "miopen_us": round(ms_lib * 1e3, 2),
+             "reference_published_us_dcu": ref_us, "reference_miopen_published_us_dcu": ref_miopen_us}
+        out.append(e)
+        log(f"reference case {e}")
+    return out
+
+
 # ------------------------------------------------------------------ fallback
 FALLBACK_SHAPES = [  # (name, n, c, h, w, k, r, s, u, v, p, q)
     ("1x1 s2 N=64 C=256 56x56 K=512", 64, 256, 56, 56, 512, 1, 1, 2, 2, 0, 0),
@@ -434,6 +492,8 @@ def main():
     ap.add_argument("--grouped-only", action="store_true", help="run the grouped workload only")
     ap.add_argument("--fallback-only", action="store_true",
                     help="run the direct-fallback workloads only")
+    ap.add_argument("--reference-cases-only", action="store_true",
+                    help="run the reference benchmark script's six shapes only")
     ap.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
     ap.add_argument("--max-batch", type=int, default=16, help="serving: requests per batch")
     ap.add_argument("--no-graph", action="store_true",
@@ -468,6 +528,9 @@ def main():
         return
     if args.fallback_only:
         print(json.dumps(bench_fallback(torch, wc, args)), flush=True)
+        return
+    if args.reference_cases_only:
+        print(json.dumps(bench_reference_cases(torch, wc, args)), flush=True)
         return
     conv, F = bench_conv(torch, wc, args)
     if args.only:  # one path and its stages alone (the profiled command)
@@ -523,6 +586,7 @@ def main():
 
     fp8 = bench_fp8(torch, wc, args)
     fallback = bench_fallback(torch, wc, args)
+    refcases = bench_reference_cases(torch, wc, args)
     grouped = bench_grouped(torch, wc, args, tuning)
     serving = bench_serving(torch, wc, args, tuning)
 
@@ -553,6 +617,7 @@ def main():
                                    if "miopen" in conv else None),
             "fp8_workload": {"shape": "N=32 C=512 H=W=14 K=512 F(4x3)", "paths": fp8},
             "fallback": fallback,
+            "reference_cases": refcases,
             "grouped": grouped,
             "serving": serving,
         },
