@@ -158,9 +158,15 @@ struct BPlan {
 // GEMM work items one batch can hold (the kernels index them in 32 bits).
 constexpr uint64_t kBatchMaxItems = (uint64_t)1 << 31;
 
+// Rows of a registered filter's U layout: K rounded to 128, or to 256 for
+// K >= 192 so that the 256-row GEMM tile (the fastest fp16 tile at every
+// K >= 256 measured, padding included) can serve it; every smaller tile
+// height divides both.
+static uint32_t engine_kp(uint32_t K) { return (uint32_t)ru(K, K >= 192 ? 256 : 128); }
+
 // A tile may only touch the K rows of the filter's U layout.
 static bool tile_fits(int cfg, int precision, uint32_t K) {
-  return gemm_cfg_ok(cfg, precision) && ru(K, gemm_cfg(cfg).bk) <= ru(K, 128);
+  return gemm_cfg_ok(cfg, precision) && ru(K, gemm_cfg(cfg).bk) <= engine_kp(K);
 }
 
 static void lookup_tile(const wino_engine* e, uint32_t m, uint32_t K, uint32_t C, uint64_t P,
@@ -172,7 +178,7 @@ static void lookup_tile(const wino_engine* e, uint32_t m, uint32_t K, uint32_t C
       *split = std::max(1, std::min<int>(te.split, (int)nsteps));
       return;
     }
-  *cfg = choose_gemm_cfg(K, C, P, t2, e->precision, (uint32_t)ru(K, 128));
+  *cfg = choose_gemm_cfg(K, C, P, t2, e->precision, engine_kp(K));
   const GemmCfg gc = gemm_cfg(*cfg);
   const uint64_t tiles = (uint64_t)((K + gc.bk - 1) / gc.bk) * ((P + gc.bp - 1) / gc.bp) * t2;
   // a problem too small to fill the device on its own splits its reduction
@@ -198,7 +204,7 @@ static int plan_batched(const wino_engine* e, uint32_t K, uint32_t C, uint32_t n
   pl.P = (uint64_t)n * pl.tiles_h * pl.tiles_w;
   pl.planes = e->precision == WINO_PREC_FP8X2 ? 2 : 1;
   pl.Cp = (uint32_t)ru(C, e->precision == WINO_PREC_FP16 ? 32 : 128);
-  pl.Kp = (uint32_t)ru(K, 128);
+  pl.Kp = engine_kp(K);
   const uint32_t nsteps = e->precision == WINO_PREC_FP16 ? pl.Cp / 32 : pl.Cp / 128;
   if (cfg < 0) lookup_tile(e, m, K, C, pl.P, pl.t2, &cfg, &split, nsteps);
   if (!tile_fits(cfg, e->precision, K) || split < 1) return WINO_ERR_INVALID;
@@ -465,7 +471,7 @@ int wino_engine_register_filter(wino_engine* e, const void* weight, uint32_t k, 
   f.K = k;
   f.C = c;
   f.algo = algo;
-  f.Kp = (uint32_t)ru(k, 128);
+  f.Kp = engine_kp(k);
   f.Cp = (uint32_t)ru(c, e->precision == WINO_PREC_FP16 ? 32 : 128);
   const bool both = algo == WINO_ALGO_AUTO || algo == WINO_ALGO_LEAST_TRAFFIC;
   const bool need2 = algo != WINO_ALGO_F43, need4 = algo == WINO_ALGO_F43 || both;

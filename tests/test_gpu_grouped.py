@@ -26,9 +26,11 @@ FP16, FP8, FP8X2 = 0, 1, 2
 TOL = {(2, FP16): 2e-3, (4, FP16): 1e-2, (2, FP8X2): 5e-3, (4, FP8X2): 1.5e-2}
 
 # filters (k, c, algo); problems (filter index, n, h, w)
-FILTERS = [(64, 32, AUTO), (40, 17, F43), (27, 3, F23), (96, 130, AUTO), (256, 40, AUTO)]
+FILTERS = [(64, 32, AUTO), (40, 17, F43), (27, 3, F23), (96, 130, AUTO), (256, 40, AUTO),
+           (320, 24, F43)]  # K = 320: U rows padded to 512 for the 256-row tile
 PROBLEMS = [(0, 1, 56, 56), (0, 2, 7, 9), (1, 3, 14, 14), (1, 1, 5, 5), (2, 2, 13, 11),
-            (2, 1, 1, 1), (3, 2, 10, 10), (3, 4, 28, 28), (0, 8, 16, 16), (4, 2, 12, 10)]
+            (2, 1, 1, 1), (3, 2, 10, 10), (3, 4, 28, 28), (0, 8, 16, 16), (4, 2, 12, 10),
+            (5, 3, 9, 11)]
 
 
 def say(*a):
@@ -105,10 +107,10 @@ def test_batch_matches_single_problem_and_fp64(wc, gpu, oracle):
         S.close()
 
 
-@pytest.mark.parametrize("j", [7, 9])
+@pytest.mark.parametrize("j", [7, 9, 10])
 def test_tile_shapes_bitwise_equal(wc, gpu, j):
     """Every tile shape the filter's K padding allows; 256-row tiles only
-    for K > 128 (problem 9, K = 256)."""
+    for K > 128 (problem 9, K = 256; problem 10, K = 320 padded to 512)."""
     S = Setup(wc, gpu, FP16)
     b = wc.Batch(S.eng, 8)
     try:
