@@ -77,6 +77,8 @@ _SIGS = {
     "wino_status_string": (C.c_char_p, [C.c_int]),
     "wino_select_algo": (C.c_int, [C.POINTER(Problem)]),
     "wino_validate": (C.c_int, [C.POINTER(Problem)]),
+    "wino_conv_error_f64": (C.c_int, [C.POINTER(Problem), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
     "wino_param_set_algo": (C.c_int, [_P, C.c_int]),
     "wino_param_set_precision": (C.c_int, [_P, C.c_int]),
     "wino_param_set_stream": (C.c_int, [_P, _P]),
@@ -166,6 +168,16 @@ def check(st: int, what: str = "") -> None:
 
 def make_problem(x, w, y, n, c, h, wd, k, r=3, s=3, u=1, v=1, p=1, q=1) -> Problem:
     return Problem(x or None, w or None, y or None, n, c, h, wd, k, r, s, u, v, p, q)
+
+
+def conv_error_f64(x: int, w: int, y: int, n, c, h, wd, k, r=3, s=3, u=1, v=1, p=1,
+                   q=1) -> tuple[float, float]:
+    """(max-abs, relative-L2) of the fp16 output at device address y against an
+    exact FP64 direct convolution of x and w, computed on the GPU."""
+    pr = make_problem(x, w, y, n, c, h, wd, k, r, s, u, v, p, q)
+    mx, rl = C.c_double(0.0), C.c_double(0.0)
+    check(lib().wino_conv_error_f64(C.byref(pr), C.byref(mx), C.byref(rl)), "conv_error_f64")
+    return mx.value, rl.value
 
 
 def select_algo(n, c, h, w, k) -> int:

@@ -126,6 +126,16 @@ int wino_stage_input(const void* param);
 int wino_stage_gemm(const void* param);
 int wino_stage_output(const void* param);
 
+/* ------------------------------------------------ error statement
+ * Max-abs and relative-L2 error of the finished fp16 output problem->out
+ * against an exact FP64 direct convolution of problem->in and
+ * problem->weight (products and sums in double, zero padding; any filter
+ * size, stride and padding), computed on the GPU for the whole tensor.  A
+ * measuring aid, called by no operator path; it runs on the null stream
+ * and returns after the result is on the host.  The reference stated
+ * errors only through its CPU check (include/verfiy.h, src/main.cpp:118-129). */
+int wino_conv_error_f64(const problem_t* problem, double* max_abs, double* rel_l2);
+
 /* ------------------------------------------------ engine: resident filters
  * An engine owns a device workspace pool (the HBM budget for per-problem
  * transformed inputs and GEMM results) and a set of registered filters whose

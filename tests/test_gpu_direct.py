@@ -153,3 +153,27 @@ def test_direct_equals_winograd_within_tolerance(wc, gpu, oracle):
     ref = oracle.conv2d_f64(x, wt)
     assert np.linalg.norm(yd - ref) / np.linalg.norm(ref) <= 1e-3
     assert np.linalg.norm(yd.astype(np.float64) - yw) / np.linalg.norm(ref) <= 1e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [(2, 5, 9, 7, 3, 5, 3, 2, 1, 2, 0), (3, 16, 12, 10, 20, 3, 3, 1, 1, 1, 1)])
+def test_gpu_fp64_error_statement_matches_oracle(wc, gpu, oracle, case):
+    """wino_conv_error_f64 (whole-tensor error against an FP64 direct
+    convolution computed on the GPU) equals the same statement computed from
+    the CPU FP64 oracle, to FP64 summation-order noise; a planted error in
+    one element moves max-abs by exactly that amount."""
+    n, c, h, w, k, r, s, u, v, p, q = case
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal((n, c, h, w)).astype(np.float16)
+    wt = (rng.standard_normal((k, c, r, s)) * 0.3).astype(np.float16)
+    ref = oracle.conv2d_f64(x, wt, u, v, p, q)
+    y = ref.astype(np.float16)
+    y.flat[7] = np.float16(float(y.flat[7]) + 0.5)
+    dx, dw, dy = gpu.to_dev(x), gpu.to_dev(wt), gpu.to_dev(y)
+    mx, rl = wc.conv_error_f64(dx.data_ptr(), dw.data_ptr(), dy.data_ptr(), n, c, h, w, k,
+                               r, s, u, v, p, q)
+    d = y.astype(np.float64) - ref
+    exp_rl = np.linalg.norm(d) / np.linalg.norm(ref)
+    assert abs(mx - np.abs(d).max()) <= 1e-12 * max(1.0, np.abs(d).max())
+    assert abs(rl - exp_rl) <= 1e-12 * exp_rl
+    assert np.abs(d).max() >= 0.49
