@@ -491,6 +491,8 @@ def main():
     ap.add_argument("--grouped-only", action="store_true", help="run the grouped workload only")
     ap.add_argument("--fallback-only", action="store_true",
                     help="run the direct-fallback workloads only")
+    ap.add_argument("--fp8-only", action="store_true",
+                    help="run the fp16 / fp8x2 / fp8 workload (N=32 C=K=512 14x14) only")
     ap.add_argument("--reference-cases-only", action="store_true",
                     help="run the reference benchmark script's six shapes only")
     ap.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
@@ -527,6 +529,9 @@ def main():
         return
     if args.fallback_only:
         print(json.dumps(bench_fallback(torch, wc, args)), flush=True)
+        return
+    if args.fp8_only:
+        print(json.dumps(bench_fp8(torch, wc, args)), flush=True)
         return
     if args.reference_cases_only:
         print(json.dumps(bench_reference_cases(torch, wc, args)), flush=True)

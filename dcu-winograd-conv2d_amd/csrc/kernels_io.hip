@@ -25,11 +25,13 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 // before it writes any of them to LDS, so SB loads per thread are in flight
 // together.  Pixels outside the image and channels past C read as zero.
 //
-// Transform: each thread handles one tile for CPT channels (2 for fp16, 4
-// for fp8).  Per tile row it reads the m+2 halves it needs as 8-byte (m = 4)
-// or 4-byte (m = 2) aligned LDS words, transforms in FP32 and stores CPT
-// consecutive channel values per tile position at once (a wave writes
-// contiguous runs of V).
+// Transform: each thread handles one tile for CPT = 2 channels (CG/2 lanes
+// per tile: 64 tiles per 256-thread pass for fp16, 16 for fp8).  Per tile
+// row it reads the m+2 halves it needs as 8-byte (m = 4) or 4-byte (m = 2)
+// aligned LDS words, transforms in FP32 and stores both channel values of a
+// tile position at once (a wave writes contiguous runs of V).  Two channels
+// per lane keep an F(4x3) thread at 2 x 36 FP32 values; four (one 32-bit
+// fp8 store per position) needed 300 registers and ran one wave per SIMD.
 // PREC: 0 fp16 (CG = 8), 1 fp8, 2 fp8 hi/lo (CG = 32).
 
 // LDS geometry of one staged window, in halves: rows, row pitch (>= ntw*m+3
@@ -58,7 +60,7 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
                                    float* __restrict__ vscale, uint32_t bx, uint32_t gb,
                                    half_t* xs, bool write_scale) {
   constexpr int T = m + 2, T2 = T * T;
-  constexpr int CPT = PREC == 0 ? 2 : 4;  // channels per thread in the transform
+  constexpr int CPT = 2;                   // channels per thread in the transform
   constexpr int SB = 8;                    // loads in flight per thread
   uint32_t b = bx;
   const uint32_t wc = b % g.nwc;
@@ -153,7 +155,7 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
   }
 
   // ---- transform
-  constexpr int LANES_PER_TILE = CG / CPT;  // 4 (fp16) or 8 (fp8)
+  constexpr int LANES_PER_TILE = CG / CPT;  // 4 (fp16) or 16 (fp8)
   const int cq = tid % LANES_PER_TILE;
   const int ntiles = (int)(nth * ntw);
   const int items = ntiles * (int)ng;  // (channel group, tile) pairs, tiles fastest
@@ -210,15 +212,14 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
       uint8_t* V = (uint8_t*)Vout;
 #pragma unroll
       for (int t = 0; t < T2; ++t) {
-        const float x4[4] = {v[0][t] * inv[t], v[1][t] * inv[t], v[2][t] * inv[t],
-                             v[3][t] * inv[t]};
+        const float x0 = v[0][t] * inv[t], x1 = v[1][t] * inv[t];
         if (PREC == 2) {
           uint32_t hi, lo;
-          e4m3x4_split(x4, hi, lo);
-          *(uint32_t*)(V + t * tplane + off) = hi;
-          *(uint32_t*)(V + (uint64_t)T2 * tplane + t * tplane + off) = lo;
+          e4m3x2_split(x0, x1, hi, lo);
+          *(uint16_t*)(V + t * tplane + off) = (uint16_t)hi;
+          *(uint16_t*)(V + (uint64_t)T2 * tplane + t * tplane + off) = (uint16_t)lo;
         } else {
-          *(uint32_t*)(V + t * tplane + off) = e4m3x4(x4[0], x4[1], x4[2], x4[3]);
+          *(uint16_t*)(V + t * tplane + off) = (uint16_t)e4m3x2(x0, x1);
         }
       }
     }
@@ -425,11 +426,11 @@ __global__ __launch_bounds__(256) void k_output(const half_t* __restrict__ M,
 void input_window(const Plan& pl, int CG, uint64_t budget, uint32_t& WR, uint32_t& WC) {
   // widest window row first (whole image rows when they fit), then as many
   // tile rows as the LDS budget allows, capped at ~4 transform passes of the
-  // block (64 tiles per pass for fp16, 32 for fp8) and at the image
+  // block (64 tiles per pass for fp16, 16 for fp8) and at the image
   WC = pl.tiles_w < 32u ? pl.tiles_w : 32u;
   while (WC > 1 && (in_lds_bytes(pl.m, CG, 1, WC) > budget || (WC * pl.m + 6) / 2 > 256))
     WC = (WC + 1) / 2;
-  const uint32_t tpp = CG == 8 ? 64u : 32u;
+  const uint32_t tpp = 512u / (uint32_t)CG;  // tiles per pass: 256 threads / (CG/2 lanes)
   uint32_t wr = (4u * tpp + WC - 1) / WC;
   if (wr > pl.tiles_h) wr = pl.tiles_h;
   if (wr < 1) wr = 1;
@@ -453,9 +454,9 @@ InputGeom input_geom(const Plan& pl, uint32_t N, uint32_t C, uint32_t H, uint32_
   g.CGp = pl.Cp / CG;
   g.NG = pad_groups ? g.CGp : div_up(C, CG);
   // A window of fewer tiles than half a transform pass (64 tiles of one
-  // group for fp16, 32 for fp8) takes several channel groups, as many as
+  // group for fp16, 16 for fp8) takes several channel groups, as many as
   // fill one pass and fit the LDS budget.
-  const uint32_t pass = CG == 8 ? 64u : 32u;
+  const uint32_t pass = 512u / (uint32_t)CG;
   const uint32_t tiles = g.WR * g.WC;
   uint32_t G = tiles < pass / 2 ? div_up(pass, tiles) : 1u;
   if (G > g.NG) G = g.NG;
@@ -521,10 +522,10 @@ hipError_t launch_output_transform(const Plan& pl, const _Float16* M, uint32_t N
 // Fixed grids; virtual block v is mapped to its problem by a forward cursor
 // over the problems' first-block offsets.
 
-// fp16: at most 128 VGPRs (4 waves per SIMD; measured faster than the
+// At most 128 VGPRs (4 waves per SIMD; for fp16 measured faster than the
 // unconstrained 144-VGPR allocation despite a small spill)
 template <int PREC, int M>
-__global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_input_grouped(const GroupHeader* __restrict__ hdr,
+__global__ __launch_bounds__(256, 4) void k_input_grouped(const GroupHeader* __restrict__ hdr,
                                                        const GProb* __restrict__ probs) {
   __shared__ __attribute__((aligned(16))) half_t gxs[kGroupInputLds / 2];
   constexpr int CG = PREC == 0 ? 8 : 32, S = M == 4 ? 1 : 0;

@@ -206,6 +206,13 @@ __device__ inline uint32_t e4m3x4(float a, float b, float c, float d) {
   return (uint32_t)r;
 }
 
+// Two values: byte 0 = a, byte 1 = b (upper 16 bits zero).
+__device__ inline uint32_t e4m3x2(float a, float b) {
+  a = __builtin_amdgcn_fmed3f(a, 448.0f, -448.0f);
+  b = __builtin_amdgcn_fmed3f(b, 448.0f, -448.0f);
+  return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xffffu;
+}
+
 // Value of byte SEL of a packed e4m3 word (hardware converter, exact).
 template <int SEL>
 __device__ inline float e4m3_at(uint32_t w) {
@@ -218,6 +225,12 @@ __device__ inline void e4m3x4_split(const float (&x)[4], uint32_t& hi, uint32_t&
   hi = e4m3x4(x[0], x[1], x[2], x[3]);
   lo = e4m3x4((x[0] - e4m3_at<0>(hi)) * 16.0f, (x[1] - e4m3_at<1>(hi)) * 16.0f,
               (x[2] - e4m3_at<2>(hi)) * 16.0f, (x[3] - e4m3_at<3>(hi)) * 16.0f);
+}
+
+// The FP8X2 planes of two values (bytes 0 and 1 of hi and lo).
+__device__ inline void e4m3x2_split(float a, float b, uint32_t& hi, uint32_t& lo) {
+  hi = e4m3x2(a, b);
+  lo = e4m3x2((a - e4m3_at<0>(hi)) * 16.0f, (b - e4m3_at<1>(hi)) * 16.0f);
 }
 
 // Smallest power of two s with amax / s <= 448 (1 for amax == 0).
