@@ -334,23 +334,39 @@ static int batch_enqueue_kernels(wino_batch* b, hipStream_t st, uint32_t mask, b
 }
 
 // Captures every kernel the batch could need into b->graph / b->exec, one
-// launch-mask bit at a time, recording the node each launch added.
+// launch-mask bit at a time, recording the node each launch added (none for
+// a bit without a kernel at this precision: the 256 x 64 GEMM tile is
+// fp16-only).  The nodes stay chained in launch order.  Making the kernels
+// of one stage independent branches (GEMM tile kernels touch disjoint
+// problems) was measured: replays of the 16-problem mix went from 227 to
+// 301 us and served capacity fell by 60 %, so the chain is kept.
 static int capture_all(wino_batch* b, hipStream_t st) {
   int rc = hipst(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   if (rc) return rc;
   const GroupHeader* hdr = (const GroupHeader*)b->dev;
   const GProb* probs = (const GProb*)(b->dev + b->probs_off);
-  if (b->eng->precision != WINO_PREC_FP16)
-    rc = hipst(launch_amax_grouped(hdr, probs, b->grid_amax, st));  // always enabled
-  for (int bit = 0; bit < 11 && !rc; ++bit) {
-    rc = batch_enqueue_kernels(b, st, 1u << bit, false);
-    if (rc) break;
+  // last node added so far (nullptr before the first)
+  auto last_node = [&](hipGraphNode_t& out) -> int {
     hipStreamCaptureStatus cs;
     const hipGraphNode_t* deps = nullptr;
     size_t ndeps = 0;
-    rc = hipst(hipStreamGetCaptureInfo_v2(st, &cs, nullptr, nullptr, &deps, &ndeps));
-    if (!rc && (cs != hipStreamCaptureStatusActive || ndeps != 1)) rc = WINO_ERR_LAUNCH;
-    if (!rc) b->node[bit] = deps[0];
+    int r = hipst(hipStreamGetCaptureInfo_v2(st, &cs, nullptr, nullptr, &deps, &ndeps));
+    if (!r && (cs != hipStreamCaptureStatusActive || ndeps > 1)) r = WINO_ERR_LAUNCH;
+    out = (!r && ndeps == 1) ? deps[0] : nullptr;
+    return r;
+  };
+  hipGraphNode_t pre = nullptr;  // the amax reduction's last node (fp8)
+  if (b->eng->precision != WINO_PREC_FP16) {
+    rc = hipst(launch_amax_grouped(hdr, probs, b->grid_amax, st));  // always enabled
+    if (!rc) rc = last_node(pre);
+  }
+  hipGraphNode_t prev = pre;
+  for (int bit = 0; bit < 11 && !rc; ++bit) {
+    rc = batch_enqueue_kernels(b, st, 1u << bit, false);
+    hipGraphNode_t cur = nullptr;
+    if (!rc) rc = last_node(cur);
+    b->node[bit] = (!rc && cur && cur != prev) ? cur : nullptr;
+    if (b->node[bit]) prev = cur;
   }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
@@ -375,8 +391,10 @@ static int batch_graph_launch(wino_batch* b, hipStream_t st) {
   const uint32_t diff = mask ^ b->enabled;
   for (int bit = 0; bit < 11; ++bit) {
     if (!(diff >> bit & 1u)) continue;
-    int rc = hipst(hipGraphNodeSetEnabled(b->exec, b->node[bit], (mask >> bit) & 1u));
-    if (rc) return rc;
+    if (b->node[bit]) {  // a bit without a captured kernel never has work
+      int rc = hipst(hipGraphNodeSetEnabled(b->exec, b->node[bit], (mask >> bit) & 1u));
+      if (rc) return rc;
+    }
     b->enabled ^= 1u << bit;
   }
   return hipst(hipGraphLaunch(b->exec, st));
@@ -644,6 +662,34 @@ static void fill_gprob(wino_batch* b, wino_batch::Entry& en, const void* in, voi
   g.amax_count = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, nin / (256 * 64)));
 }
 
+// The grouped GEMM runs one persistent kernel per tile configuration in use,
+// one after another, so a configuration used only by a problem of a few
+// tiles costs a whole kernel of launch and pipeline latency (8.7 us for
+// N=1 C=K=128 28x28 among 15 larger problems).  Such a problem (at most
+// kSmallItems GEMM tiles, no reduction split, tile configuration not yet
+// in the batch) takes the configuration of the batch's largest GEMM work
+// that fits its filter instead.  The result is bitwise the same: every
+// tile configuration reduces over C in the same order.
+constexpr uint64_t kSmallItems = 128;
+
+static bool merge_small_tile(const wino_batch* b, const wino_engine::Filter& f, uint32_t n,
+                             uint32_t h, uint32_t w, int m, const BPlan& own, BPlan* out) {
+  if (own.items > kSmallItems || own.split != 1) return false;
+  uint64_t work[kGemmCfgs] = {};
+  for (const auto& o : b->entries) {
+    if (o.bp.pl.gemm_cfg == own.pl.gemm_cfg) return false;  // its kernel runs anyway
+    work[o.bp.pl.gemm_cfg] += o.bp.items * gemm_cfg(o.bp.pl.gemm_cfg).bk *
+                              gemm_cfg(o.bp.pl.gemm_cfg).bp;
+  }
+  int best = -1;
+  for (int c = 0; c < kGemmCfgs; ++c)
+    if (work[c] && tile_fits(c, b->eng->precision, f.K) && (best < 0 || work[c] > work[best]))
+      best = c;
+  if (best < 0) return false;
+  return plan_batched(b->eng, f.K, f.C, n, h, w, m, best, 1, out) == WINO_OK &&
+         out->footprint <= b->eng->pool.capacity() && out->items <= b->max_items;
+}
+
 int wino_batch_add(wino_batch* b, int filter_id, const void* in, void* out, uint32_t n,
                    uint32_t h, uint32_t w, int algo, int* index) {
   if (!b || !in || !out || !n || !h || !w) return WINO_ERR_INVALID;
@@ -661,10 +707,16 @@ int wino_batch_add(wino_batch* b, int filter_id, const void* in, void* out, uint
   if (st) return st;
   if (en.bp.footprint > e->pool.capacity() || en.bp.items > b->max_items)
     return WINO_ERR_UNSUPPORTED;  // can never fit, whatever else is running
-  uint64_t items = en.bp.items;
-  for (const auto& o : b->entries) items += o.bp.items;
-  if (items > b->max_items) return WINO_ERR_CAPACITY;
-  if (!e->pool.alloc(en.bp.footprint, &en.pool_off)) return WINO_ERR_CAPACITY;
+  uint64_t others = 0;
+  for (const auto& o : b->entries) others += o.bp.items;
+  BPlan merged;
+  if (merge_small_tile(b, f, n, h, w, m, en.bp, &merged) &&
+      others + merged.items <= b->max_items && e->pool.alloc(merged.footprint, &en.pool_off)) {
+    en.bp = merged;
+  } else {
+    if (others + en.bp.items > b->max_items) return WINO_ERR_CAPACITY;
+    if (!e->pool.alloc(en.bp.footprint, &en.pool_off)) return WINO_ERR_CAPACITY;
+  }
   en.has_ws = true;
   fill_gprob(b, en, in, out, n, h, w);
   b->entries.push_back(en);

@@ -210,14 +210,17 @@ def test_split_reduction_deterministic(wc, gpu, oracle):
         S.close()
 
 
-def test_graph_replay_new_contents(wc, gpu):
+@pytest.mark.parametrize("precision,force", [(FP16, None), (FP8X2, 4)])
+def test_graph_replay_new_contents(wc, gpu, precision, force):
     """One capture, several replays with different batch contents and eager
-    work between them; every replay equals the eager launch.  Each mix is
-    replayed three times (the all-kernels graph, then the capture of its own
-    kernel set on the second sighting, then that cached graph) and the
-    sequence of mixes runs twice, so cached sets are reused after others."""
+    work between them; every replay equals the eager launch, bitwise.  Each
+    mix is replayed three times and the sequence of mixes runs twice, so the
+    graph's kernel nodes are disabled and re-enabled in many combinations.
+    `force` puts the first problem of each mix on that GEMM tile (fp8 has no
+    256 x 64 tile kernel, so its graph has no node for that tile and the
+    16 x 64 tile's node must not be taken for it)."""
     import torch
-    S = Setup(wc, gpu, FP16)
+    S = Setup(wc, gpu, precision)
     eager = wc.Batch(S.eng, 16)
     graph = wc.Batch(S.eng, 16)
     stream = torch.cuda.Stream()
@@ -229,16 +232,20 @@ def test_graph_replay_new_contents(wc, gpu):
             eager.reset()
             for j, a in zip(js, ye):
                 S.add(eager, j, a)
+            if force is not None:
+                eager.set_tile(0, force)
             eager.launch()
             for rep in range(3):
                 yg = [S.fresh_out(j) for j in js]
                 graph.reset()
                 for j, g in zip(js, yg):
                     S.add(graph, j, g)
+                if force is not None:
+                    graph.set_tile(0, force)
                 if not captured:
                     graph.capture(stream.cuda_stream)
                     captured = True
-                say(f"graph replay {r}.{rep}: problems {js}")
+                say(f"graph replay precision={precision} force={force} {r}.{rep}: problems {js}")
                 graph.launch(stream.cuda_stream)
                 stream.synchronize()
                 # eager work between replays
