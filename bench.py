@@ -160,12 +160,16 @@ def bench_conv(torch, wc, args):
 
 
 def dominant_stage(torch, wc, op, args):
-    """Times each non-fused stage of `op` alone (HIP events on the null
-    stream, the stream the stages are launched on); returns the slowest."""
+    """Average duration (us) of each non-fused stage of `op`, launched alone
+    `steps` times in a row between two HIP events on the null stream (the
+    stream the stages are launched on).  Events between the stages of one
+    pipeline pass were measured to add ~3 us per kernel (each record
+    releases the caches), so a stage is timed over a loop of its own
+    launches.  rocprof's kernel trace of `bench.py --only f43` gives the
+    GEMM the same average inside the pipeline as in such a loop."""
     inf = op.info()
-    stages = ["input", "gemm", "output"]
     out = {}
-    for s in stages:
+    for s in ("input", "gemm", "output"):
         out[s] = time_loop(torch, lambda s=s: op.stage(s), args.steps, args.warmup) * 1e3
     return out, inf
 
