@@ -42,6 +42,16 @@ struct TileShape {
 };
 
 // One BK x BP tile of M[t] over reduction steps [s0, s1) (32 channels each).
+//
+// Operand fragments are staged through a two-buffer LDS ring.  Global loads
+// run two steps ahead in two alternating register sets, and every load is a
+// bounds-checked buffer load whose range ends at step s1: the steps past the
+// end (the odd step of an odd count, and the look-ahead of the last two)
+// read zeros without a branch, so the compiler keeps the newer loads in
+// flight when it waits for the older ones.  With one set one step ahead, a
+// tile of few steps (C = 64 or 128) paid a full memory latency per step.
+// A zero step adds +0 to every accumulator (which starts at +0): the
+// result is bitwise that of the exact step count.
 template <int WK, int WP, int TK, int TP>
 __device__ __attribute__((always_inline)) inline void gemm_tile_f16(char* smem, const half8* __restrict__ U,
                                      const half8* __restrict__ V, half_t* __restrict__ M,
@@ -58,23 +68,41 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16(char* smem, 
   const int wk = wave % WK, wp = wave / WK;
   const uint32_t k0 = kb * BK;
   const uint64_t p0 = pb * BP;
-  const half8* Ut = U + (uint64_t)t * C8p * Kp + k0;
-  const half8* Vt = V + (uint64_t)t * C8p * Pp + p0;
+  // Byte ranges end at step s1 of this tile's rows / columns (entries of
+  // step s start at 4 s Kp resp. 4 s Pp); < 2^32 since C Kp and C Pp are.
+  const __amdgpu_buffer_rsrc_t ra_rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(U + (uint64_t)t * C8p * Kp + k0), 0, (int)(uint32_t)(64u * (uint32_t)s1 * Kp), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rb_rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(V + (uint64_t)t * C8p * Pp + p0), 0, (int)(uint32_t)(64u * (uint32_t)s1 * (uint32_t)Pp),
+      0x00020000);
+  // per-thread byte offsets of step 0; step s adds 64 s Kp resp. 64 s Pp
+  uint32_t oa[ALD], ob[BLD];
+#pragma unroll
+  for (int i = 0; i < ALD; ++i) {
+    const int q = tid + i * NT;
+    oa[i] = 16u * ((uint32_t)(q / BK) * Kp + (uint32_t)(q % BK));
+  }
+#pragma unroll
+  for (int i = 0; i < BLD; ++i) {
+    const int q = tid + i * NT;
+    ob[i] = 16u * ((uint32_t)(q / BP) * (uint32_t)Pp + (uint32_t)(q % BP));
+  }
+  const uint32_t sa = 64u * Kp, sb = 64u * (uint32_t)Pp;
 
-  half8 ra[ALD], rb[BLD];
-  auto gload = [&](int s) {
+  typedef half8 aset_t[ALD];
+  typedef half8 bset_t[BLD];
+  half8 ra0[ALD], rb0[BLD], ra1[ALD], rb1[BLD];
+  auto gload = [&](int s, aset_t& ra, bset_t& rb) {
 #pragma unroll
-    for (int i = 0; i < ALD; ++i) {
-      const int q = tid + i * NT;
-      if (q < 4 * BK) ra[i] = Ut[(uint64_t)(4 * s + q / BK) * Kp + (q % BK)];
-    }
+    for (int i = 0; i < ALD; ++i)
+      ra[i] = __builtin_bit_cast(
+          half8, __builtin_amdgcn_raw_buffer_load_b128(ra_rs, oa[i] + (uint32_t)s * sa, 0, 0));
 #pragma unroll
-    for (int i = 0; i < BLD; ++i) {
-      const int q = tid + i * NT;
-      if (q < 4 * BP) rb[i] = Vt[(uint64_t)(4 * s + q / BP) * Pp + (q % BP)];
-    }
+    for (int i = 0; i < BLD; ++i)
+      rb[i] = __builtin_bit_cast(
+          half8, __builtin_amdgcn_raw_buffer_load_b128(rb_rs, ob[i] + (uint32_t)s * sb, 0, 0));
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, const aset_t& ra, const bset_t& rb) {
 #pragma unroll
     for (int i = 0; i < ALD; ++i) {
       const int q = tid + i * NT;
@@ -94,23 +122,31 @@ __device__ __attribute__((always_inline)) inline void gemm_tile_f16(char* smem, 
     for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   const int g = lane >> 4, r16 = lane & 15;
-  gload(s0);
-  sstore(0);
-  __syncthreads();
-  for (int s = s0; s < s1; ++s) {
-    const int cur = (s - s0) & 1;
-    if (s + 1 < s1) gload(s + 1);
+  auto compute = [&](int buf) {
     half8 a[TK], b[TP];
 #pragma unroll
-    for (int i = 0; i < TK; ++i) a[i] = As[(cur * 4 + g) * BK + wk * TK * 16 + i * 16 + r16];
+    for (int i = 0; i < TK; ++i) a[i] = As[(buf * 4 + g) * BK + wk * TK * 16 + i * 16 + r16];
 #pragma unroll
-    for (int j = 0; j < TP; ++j) b[j] = Bs[(cur * 4 + g) * BP + wp * TP * 16 + j * 16 + r16];
+    for (int j = 0; j < TP; ++j) b[j] = Bs[(buf * 4 + g) * BP + wp * TP * 16 + j * 16 + r16];
 #pragma unroll
     for (int i = 0; i < TK; ++i)
 #pragma unroll
       for (int j = 0; j < TP; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
-    if (s + 1 < s1) sstore(cur ^ 1);
+  };
+  // steps s0 + 2j sit in set 0 / LDS buffer 0, s0 + 2j + 1 in set 1 / buffer 1
+  gload(s0, ra0, rb0);
+  gload(s0 + 1, ra1, rb1);
+  sstore(0, ra0, rb0);
+  __syncthreads();
+  for (int s = s0; s < s1; s += 2) {
+    gload(s + 2, ra0, rb0);
+    compute(0);
+    sstore(1, ra1, rb1);
+    __syncthreads();
+    gload(s + 3, ra1, rb1);
+    compute(1);
+    sstore(0, ra0, rb0);
     __syncthreads();
   }
 
