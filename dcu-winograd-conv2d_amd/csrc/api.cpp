@@ -97,7 +97,8 @@ int choose_gemm_cfg(uint32_t K, uint32_t C, uint64_t P, int t2, int precision, u
 
 int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
   memset(pl, 0, sizeof(*pl));
-  if (algo == WINO_ALGO_AUTO) algo = winograd_ok(pr) ? reference_select(pr) : WINO_ALGO_DIRECT;
+  const bool auto_algo = algo == WINO_ALGO_AUTO;
+  if (auto_algo) algo = winograd_ok(pr) ? reference_select(pr) : WINO_ALGO_DIRECT;
   if (algo < WINO_ALGO_F43 || algo > WINO_ALGO_DIRECT) return WINO_ERR_INVALID;
   if (precision < WINO_PREC_FP16 || precision > WINO_PREC_FP8X2) return WINO_ERR_INVALID;
   if (algo == WINO_ALGO_DIRECT) {
@@ -161,6 +162,14 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
     pl->u_bytes = (uint64_t)pl->planes * pl->t2 * pl->Cp * pl->Kp;
     pl->v_bytes = (uint64_t)pl->planes * pl->t2 * pl->Cp * pl->Pp;
     pl->scale_bytes = (uint64_t)pl->t2 * pl->Kp * 4 + 256 + 256;
+  }
+  // The LDS-staged GEMM tiles address one tile position's plane of U or V
+  // with 32-bit byte offsets.  Only problems whose V alone is 64 GB or more
+  // exceed that; AUTO gives them to the direct convolution.
+  if (!plane_offsets_fit(pl->Cp, pl->Kp, pl->Pp)) {
+    if (auto_algo && precision == WINO_PREC_FP16)
+      return make_plan(pr, WINO_ALGO_DIRECT, precision, pl);
+    return WINO_ERR_UNSUPPORTED;
   }
   pl->m_bytes = (uint64_t)pl->t2 * pl->Kp * pl->Pp * 2;
   uint64_t off = 0;

@@ -214,6 +214,7 @@ static int plan_batched(const wino_engine* e, uint32_t K, uint32_t C, uint32_t n
   pl.bk = gc.bk;
   pl.bp = gc.bp;
   pl.Pp = ru(pl.P, gc.bp);
+  if (!plane_offsets_fit(pl.Cp, pl.Kp, pl.Pp)) return WINO_ERR_UNSUPPORTED;
   bp->kblocks = (K + gc.bk - 1) / gc.bk;
   bp->split = (uint32_t)split;
   const uint64_t vb = e->precision == WINO_PREC_FP16

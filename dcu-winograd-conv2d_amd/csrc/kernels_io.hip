@@ -79,6 +79,13 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
   const int NCT = (int)ng * CG;  // channels staged
 
   // ---- stage
+  // Thread -> (staged row lr = slot + j * rpp, dword column dcol), fixed;
+  // it walks the channels.  Each channel has its own buffer resource (base =
+  // the channel's plane, range = the plane, or 0 for a channel past C),
+  // built with scalar instructions; the lane's byte offset within a plane is
+  // computed once per row, or set past every range when the row or column
+  // lies outside the image, so a load reads 0 there and costs no vector
+  // instruction.  SB loads are in flight per thread before their LDS writes.
   {
     const int LWd = L.LWp / 2;  // dwords per LDS row (<= 256 by construction)
     const int rpp = 256 / LWd;  // rows per pass
@@ -88,56 +95,38 @@ __device__ __attribute__((always_inline)) inline void input_block(const half_t* 
     const uint32_t c0 = cg0 * CG;
     const uint32_t nch = c0 < g.C ? min((uint32_t)NCT, g.C - c0) : 0u;
     const uint64_t plane = (uint64_t)g.H * g.W;
-    // byte range of the group's real channels: < 2^32 (tensors hold < 2^31 elements)
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(X + ((uint64_t)n * g.C + (nch ? c0 : 0u)) * plane), 0,
-        (int)(uint32_t)(nch * plane * 2u), 0x00020000);
+    const uint32_t plane2 = (uint32_t)plane * 2u, W2 = g.W * 2u;  // < 2^32: tensors < 2^31 elements
+    const uint64_t gbase =
+        (uint64_t)(uintptr_t)X + 2ull * ((uint64_t)n * g.C + (nch ? c0 : 0u)) * plane;
+    const int psd = L.PS / 2;  // LDS dwords per channel slot
+    constexpr uint32_t kOff = 0xfffffff0u;  // past every range: reads 0
     uint32_t* xs32 = (uint32_t*)xs;
-    // Row r = ch * LH + lr of the staged image advances by rpp rows per
-    // load; the global byte offset (32-bit: the buffer range is < 2^32, and
-    // offsets of masked rows may wrap harmlessly) and the LDS dword index
-    // advance by constants, with one correction when lr wraps past LH.
-    const uint32_t plane2 = (uint32_t)plane * 2u, W2 = g.W * 2u;
-    const int dch = rpp / L.LH, dlr = rpp % L.LH;
-    const uint32_t g_step = (uint32_t)dch * plane2 + (uint32_t)dlr * W2;
-    const uint32_t g_wrap = plane2 - (uint32_t)L.LH * W2;
-    const int l_step = (dch * L.PS + dlr * L.LWp) / 2, l_wrap = (L.PS - L.LH * L.LWp) / 2;
-    const int lr_lo = y0 < 0 ? -y0 : 0, lr_hi = min(L.LH, (int)g.H - y0);
     if (slot < rpp) {
-      int ch = slot / L.LH, lr = slot % L.LH;
-      uint32_t goff = (uint32_t)ch * plane2 + (uint32_t)(y0 + lr) * W2 + 2u * (uint32_t)x;
-      int loff = (ch * L.PS + lr * L.LWp) / 2 + dcol;
-      while (ch < NCT) {
-        uint32_t val[SB];
-        int dst[SB];
+      for (int lr = slot; lr < L.LH; lr += rpp) {
+        const int y = y0 + lr;
+        const bool yin = y >= 0 && y < (int)g.H;
+        const uint32_t ro = (uint32_t)y * W2 + 2u * (uint32_t)x;
+        const uint32_t o0 = yin && x0in ? ro : kOff, o1 = yin && x1in ? ro + 2u : kOff;
+        uint32_t* dst = xs32 + lr * LWd + dcol;
+        for (int cb = 0; cb < NCT; cb += SB) {
+          uint32_t val[SB];
 #pragma unroll
-        for (int u = 0; u < SB; ++u) {
-          const bool iny = (uint32_t)ch < nch && lr >= lr_lo && lr < lr_hi;
-          if constexpr (EVENW) {
-            const bool in = iny && x0in;
-            const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rs, in ? goff : 0u, 0, 0);
-            val[u] = in ? v : 0u;
-          } else {
-            const bool in0 = iny && x0in, in1 = iny && x1in;
-            const uint32_t a = __builtin_amdgcn_raw_buffer_load_b16(rs, in0 ? goff : 0u, 0, 0);
-            const uint32_t c = __builtin_amdgcn_raw_buffer_load_b16(rs, in1 ? goff + 2u : 0u, 0, 0);
-            val[u] = (in0 ? (a & 0xffffu) : 0u) | ((in1 ? (c & 0xffffu) : 0u) << 16);
+          for (int u = 0; u < SB; ++u) {
+            const uint32_t ch = (uint32_t)(cb + u);
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(uintptr_t)(gbase + (uint64_t)ch * plane2), 0,
+                ch < nch ? (int)plane2 : 0, 0x00020000);
+            if constexpr (EVENW) {
+              val[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, o0, 0, 0);
+            } else {
+              const uint32_t a = __builtin_amdgcn_raw_buffer_load_b16(rs, o0, 0, 0);
+              const uint32_t c = __builtin_amdgcn_raw_buffer_load_b16(rs, o1, 0, 0);
+              val[u] = (a & 0xffffu) | (c << 16);
+            }
           }
-          dst[u] = ch < NCT ? loff : -1;
-          ch += dch;
-          lr += dlr;
-          goff += g_step;
-          loff += l_step;
-          if (lr >= L.LH) {
-            lr -= L.LH;
-            ++ch;
-            goff += g_wrap;
-            loff += l_wrap;
-          }
+#pragma unroll
+          for (int u = 0; u < SB; ++u) dst[(cb + u) * psd] = val[u];
         }
-#pragma unroll
-        for (int u = 0; u < SB; ++u)
-          if (dst[u] >= 0) xs32[dst[u]] = val[u];
       }
     }
   }

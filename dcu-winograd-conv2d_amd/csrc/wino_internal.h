@@ -62,6 +62,12 @@ int reference_select(const problem_t* pr);
 int validate(const problem_t* pr);
 bool winograd_ok(const problem_t* pr);  // 3x3, stride 1, pad 1
 int choose_gemm_cfg(uint32_t K, uint32_t C, uint64_t P, int t2, int precision, uint32_t kmax);
+// One tile position's plane of U (Cp x Kp) and of V (Cp x Pp), fp16, is
+// below 4 GiB: the LDS-staged GEMM tiles address a plane through buffer
+// resources with 32-bit byte offsets.
+inline bool plane_offsets_fit(uint64_t Cp, uint64_t Kp, uint64_t Pp) {
+  return Cp * Pp * 2 < (1ull << 32) && Cp * Kp * 2 < (1ull << 32);
+}
 
 // ------------------------------------------------------------ GEMM table
 struct GemmCfg {
