@@ -121,6 +121,13 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
   if (!winograd_ok(pr)) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F43_FUSED) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F23_FUSED && precision != WINO_PREC_FP16) return WINO_ERR_UNSUPPORTED;
+  // The fused kernel's input offsets are 32-bit below a sentinel: input of
+  // at most 1 GiB.  Larger problems the reference selector gives to it run
+  // the non-fused F(2x3) under AUTO.
+  if (algo == WINO_ALGO_F23_FUSED && !fused_input_fits(pr)) {
+    if (!auto_algo) return WINO_ERR_UNSUPPORTED;
+    algo = WINO_ALGO_F23;
+  }
   pl->algo = algo;
   pl->precision = precision;
   pl->m = (algo == WINO_ALGO_F43 || algo == WINO_ALGO_F43_FUSED) ? 4 : 2;

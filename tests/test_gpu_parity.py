@@ -244,6 +244,30 @@ def test_auto_selection_follows_reference(wc, gpu):
         assert np.isfinite(y).all()
 
 
+def test_auto_fused_input_limit(wc, gpu):
+    """The fused F(2x3) kernel takes inputs of at most 1 GiB.  The reference
+    selector gives N=128 C=64 258x258 K=64 (input 1.02 GiB) to it; AUTO then
+    plans the non-fused F(2x3) instead, and forcing the fused algorithm is
+    refused.  The same shape at N=2 keeps the fused kernel.  Planning only:
+    no tensor is read or written, so the addresses are 0."""
+    n, c, h, k = 128, 64, 258, 64
+    assert 2 * n * c * h * h > 1 << 30
+    assert wc.select_algo(n, c, h, h, k) == F23F
+    op = wc.Conv2d(0, 0, 0, n, c, h, h, k)
+    try:
+        assert op.info().algo == F23
+    finally:
+        op.destroy()
+    with pytest.raises(wc.WinoError) as e:
+        wc.Conv2d(0, 0, 0, n, c, h, h, k, algo=F23F)
+    assert e.value.status == wc.ERR_UNSUPPORTED
+    op = wc.Conv2d(0, 0, 0, 2, c, h, h, k)
+    try:
+        assert op.info().algo == F23F
+    finally:
+        op.destroy()
+
+
 @pytest.mark.parametrize("algo,prec", [(F43, FP16), (F23F, FP16), (F43, FP8X2), (F23, FP16)])
 def test_n64_c256_28x28_k256_sampled(wc, gpu, oracle, algo, prec):
     """N=64 C=256 28x28 K=256 (the throughput configuration), 1% sample."""
