@@ -459,6 +459,139 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
                                   (int)(C8p / 4), 0);
 }
 
+// Persistent form of the 256 x 64 rows tile for one problem whose reduction
+// has NS 32-channel steps (NS a multiple of 4).  A block takes the tiles
+// lo + slot, lo + slot + nbx, ... of its XCD's contiguous tile range, and
+// the register rings run over the concatenated steps of its tiles: A two
+// steps ahead, B four steps ahead, so the last steps of a tile already load
+// the first steps of the next, and those loads are in flight during the
+// tile's epilogue.  Every ring slot and every "this tile or the next"
+// choice is static (the tile loop body is the NS unrolled steps).  The B
+// ring and the epilogue tile use separate LDS, so the epilogue needs no
+// barrier against the next tile's first steps.  Loads past the block's last
+// tile re-read its last step (L2 hits, never used).
+template <int NS>
+__global__ __launch_bounds__(256, 3) void k_gemm_rows_p(const half8* __restrict__ U,
+                                                     const half8* __restrict__ V,
+                                                     half_t* __restrict__ M, uint32_t Kp,
+                                                     uint64_t Pp, uint32_t C8p, uint32_t ntiles,
+                                                     uint32_t gx, uint32_t gy) {
+  static_assert(NS % 4 == 0, "step count: a multiple of 4");
+  constexpr int TK = 4, TP = 4, BK = 256, BP = 64, CS = BP + 8;
+  __shared__ __attribute__((aligned(16))) half8 Bs[2 * 4 * BP];    // B ring, 8 KB
+  __shared__ __attribute__((aligned(16))) half_t Cs[BK * CS];      // epilogue tile, 36 KB
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int g = lane >> 4, r16 = lane & 15;
+  const int bq_g = tid / BP, bq_c = tid % BP;  // this thread's B chunk (one per step)
+
+  // XCD x (blocks b with b % 8 == x; the grid is a multiple of 8) owns tiles
+  // [lo, hi); tile index order keeps the tile position t slowest, so each
+  // U[t] is read by one or two XCDs' L2
+  const uint32_t xcd = blockIdx.x % 8u, slot = blockIdx.x / 8u, nbx = gridDim.x / 8u;
+  const uint32_t per = (ntiles + 7u) / 8u;
+  const uint32_t lo = min(xcd * per, ntiles), hi = min(lo + per, ntiles);
+  const uint32_t mine = lo + slot < hi ? (hi - lo - slot + nbx - 1u) / nbx : 0u;
+  if (mine == 0) return;  // block-uniform
+
+  struct Tile {
+    const half8* ua;  // this lane's A row base
+    const half8* vb;  // this thread's B chunk base
+    uint32_t t, kb, pb;
+  };
+  auto tile = [&](uint32_t i) {
+    Tile c;
+    const uint32_t L = lo + slot + i * nbx;
+    c.pb = L % gx;
+    const uint32_t r = L / gx;
+    c.kb = r % gy;
+    c.t = r / gy;
+    c.ua = U + (uint64_t)c.t * C8p * Kp + (uint64_t)g * Kp + c.kb * BK + wave * 64 + r16;
+    c.vb = V + (uint64_t)c.t * C8p * Pp + (uint64_t)c.pb * BP + (uint64_t)bq_g * Pp + bq_c;
+    return c;
+  };
+  auto load_a = [&](const half8* src, half8 (&a)[TK]) {
+#pragma unroll
+    for (int i = 0; i < TK; ++i) a[i] = src[i * 16];
+  };
+
+  floatx4 acc[TK][TP];
+#pragma unroll
+  for (int i = 0; i < TK; ++i)
+#pragma unroll
+    for (int j = 0; j < TP; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  // A(x) in ra[x % 2], B(x) in rb[x % 4] (x counts the block's steps).
+  // pa / pv point at the next A / B step to load; they advance by one step
+  // per load and move to the next tile at its first step (or stay on the
+  // last step after the block's last tile).
+  const uint64_t sa = 4ull * Kp, sv = 4ull * Pp;  // half8 entries per step
+  half8 ra[2][TK], rb[4];
+  Tile cur = tile(0);
+  load_a(cur.ua, ra[0]);
+  rb[0] = cur.vb[0];
+  rb[1] = cur.vb[sv];
+  load_a(cur.ua + sa, ra[1]);
+  rb[2] = cur.vb[2 * sv];
+  rb[3] = cur.vb[3 * sv];
+  const half8* pa = cur.ua + 2 * sa;
+  const half8* pv = cur.vb + 4 * sv;
+  uint64_t ia = sa, iv = sv;
+  Bs[tid] = rb[0];
+  lds_barrier();
+
+  for (uint32_t i = 0; i < mine; ++i) {
+    const bool more = i + 1 < mine;
+    const Tile nxt = more ? tile(i + 1) : cur;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int par = s & 1;
+      half8 b[TP];
+#pragma unroll
+      for (int j = 0; j < TP; ++j) b[j] = Bs[(par * 4 + g) * BP + j * 16 + r16];
+      // transposed product (A operand = V, B operand = U): lane ends with 4
+      // consecutive tiles of one output channel
+#pragma unroll
+      for (int q = 0; q < TK; ++q)
+#pragma unroll
+        for (int j = 0; j < TP; ++j)
+          acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[j], ra[s & 1][q], acc[q][j], 0, 0, 0);
+      Bs[(par ^ 1) * 4 * BP + tid] = rb[(s + 1) & 3];  // B(s+1)
+      if (s + 2 == NS) {  // A(s+2) is the next tile's first step
+        pa = more ? nxt.ua : pa - sa;
+        ia = more ? sa : 0;
+      }
+      load_a(pa, ra[s & 1]);
+      pa += ia;
+      if (s + 4 == NS) {  // B(s+4) is the next tile's first step
+        pv = more ? nxt.vb : pv - sv;
+        iv = more ? sv : 0;
+      }
+      rb[s & 3] = *pv;
+      pv += iv;
+      lds_barrier();
+    }
+    // epilogue: acc[q][j][e] is channel kb*256 + 64 wave + 16 q + r16,
+    // tile pb*64 + 16 j + 4 g + e
+#pragma unroll
+    for (int q = 0; q < TK; ++q)
+#pragma unroll
+      for (int j = 0; j < TP; ++j) {
+        *(half4*)(Cs + (wave * 64 + q * 16 + r16) * CS + j * 16 + 4 * g) =
+            half4{(half_t)acc[q][j][0], (half_t)acc[q][j][1], (half_t)acc[q][j][2],
+                  (half_t)acc[q][j][3]};
+        acc[q][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+    lds_barrier();
+    half_t* Mt = M + ((uint64_t)cur.t * Kp + cur.kb * BK) * Pp + (uint64_t)cur.pb * BP;
+    for (int q = tid; q < BK * BP / 8; q += 256) {
+      const int row = q / (BP / 8), c8 = q % (BP / 8);
+      *(half8*)(Mt + (uint64_t)row * Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
+    }
+    cur = nxt;
+  }
+}
+
 template <int WK, int WP, int TK, int TP, int PLANES, bool LOLO>
 __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f8(
     const intx8* __restrict__ U, const intx8* __restrict__ V, half_t* __restrict__ M,
@@ -502,9 +635,41 @@ GemmCfg gemm_cfg(int idx) {
   return c;
 }
 
+// The 256 x 64 tile as a persistent kernel (k_gemm_rows_p) when the step
+// count has an instance; false otherwise.
+static bool run_rows_persistent(const Plan& pl, const void* U, const void* V, half_t* M,
+                                hipStream_t st) {
+  const uint32_t ns = pl.Cp / 32, gx = (uint32_t)(pl.Pp / 64), gy = pl.Kp / 256;
+  const uint32_t ntiles = gx * gy * (uint32_t)pl.t2;
+  static int cus = 0;
+  if (cus <= 0) {
+    int dev = 0, n = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    cus = n > 0 ? n : 256;
+  }
+  // 3 blocks per CU, a multiple of 8 blocks (one share per XCD)
+  uint32_t grid = ((uint32_t)cus * 3u) & ~7u;
+  const uint32_t need = (ntiles + 7u) & ~7u;
+  if (grid > need) grid = need;
+  if (grid < 8u) grid = 8u;
+#define WINO_ROWS_P(NSV)                                                                    \
+  hipLaunchKernelGGL(k_gemm_rows_p<NSV>, dim3(grid), dim3(256), 0, st, (const half8*)U,      \
+                     (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8, ntiles, gx, gy)
+  switch (ns) {
+    case 4: WINO_ROWS_P(4); return true;
+    case 8: WINO_ROWS_P(8); return true;
+    case 16: WINO_ROWS_P(16); return true;
+    default: return false;
+  }
+#undef WINO_ROWS_P
+}
+
 template <int WK, int WP, int TK, int TP>
 static void run_f16(const Plan& pl, const void* U, const void* V, half_t* M, hipStream_t st) {
   constexpr int BK = WK * TK * 16, BP = WP * TP * 16;
+  if constexpr (WK == 4 && WP == 1 && TK == 4 && TP == 4)
+    if (run_rows_persistent(pl, U, V, M, st)) return;
   dim3 grid((uint32_t)(pl.Pp / BP), pl.Kp / BK, pl.t2);
   hipLaunchKernelGGL((k_gemm_f16<WK, WP, TK, TP>), grid, dim3(WK * WP * 64), 0, st,
                      (const half8*)U, (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8,

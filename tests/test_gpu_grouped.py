@@ -26,11 +26,14 @@ FP16, FP8, FP8X2 = 0, 1, 2
 TOL = {(2, FP16): 2e-3, (4, FP16): 1e-2, (2, FP8X2): 5e-3, (4, FP8X2): 1.5e-2}
 
 # filters (k, c, algo); problems (filter index, n, h, w)
+# K = 320: U rows padded to 512 for the 256-row tile.  K = 256 with C = 128
+# and C = 512: the single-problem operator runs the persistent 256-row GEMM
+# (4 and 16 reduction steps), compared bitwise with the batch's tiles.
 FILTERS = [(64, 32, AUTO), (40, 17, F43), (27, 3, F23), (96, 130, AUTO), (256, 40, AUTO),
-           (320, 24, F43)]  # K = 320: U rows padded to 512 for the 256-row tile
+           (320, 24, F43), (256, 128, F43), (256, 512, F43)]
 PROBLEMS = [(0, 1, 56, 56), (0, 2, 7, 9), (1, 3, 14, 14), (1, 1, 5, 5), (2, 2, 13, 11),
             (2, 1, 1, 1), (3, 2, 10, 10), (3, 4, 28, 28), (0, 8, 16, 16), (4, 2, 12, 10),
-            (5, 3, 9, 11)]
+            (5, 3, 9, 11), (6, 2, 12, 12), (7, 1, 9, 9)]
 
 
 def say(*a):
