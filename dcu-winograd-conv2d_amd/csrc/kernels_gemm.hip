@@ -460,23 +460,22 @@ __global__ __launch_bounds__(WK* WP * 64) void k_gemm_f16(const half8* __restric
 }
 
 // Persistent form of the 256 x 64 rows tile for one problem whose reduction
-// has NS 32-channel steps (NS a multiple of 4).  A block takes the tiles
-// lo + slot, lo + slot + nbx, ... of its XCD's contiguous tile range, and
-// the register rings run over the concatenated steps of its tiles: A two
-// steps ahead, B four steps ahead, so the last steps of a tile already load
-// the first steps of the next, and those loads are in flight during the
-// tile's epilogue.  Every ring slot and every "this tile or the next"
-// choice is static (the tile loop body is the NS unrolled steps).  The B
-// ring and the epilogue tile use separate LDS, so the epilogue needs no
-// barrier against the next tile's first steps.  Loads past the block's last
-// tile re-read its last step (L2 hits, never used).
-template <int NS>
+// has a multiple of 4 (at least 4) 32-channel steps.  A block takes the
+// tiles lo + slot, lo + slot + nbx, ... of its XCD's contiguous tile range,
+// and the register rings run over the concatenated steps of its tiles: A
+// two steps ahead, B four steps ahead, so the last steps of a tile already
+// load the first steps of the next, and those loads are in flight during
+// the tile's epilogue.  Steps run in unrolled groups of 4, so every ring
+// slot is static; only a tile's last group holds the "next tile" choices.
+// The B ring and the epilogue tile use separate LDS, so the epilogue needs
+// no barrier against the next tile's first steps.  Loads past the block's
+// last tile re-read its last step (L2 hits, never used).  At most 168
+// registers (3 blocks per CU); the accumulators then live in VGPRs.
 __global__ __launch_bounds__(256, 3) void k_gemm_rows_p(const half8* __restrict__ U,
-                                                     const half8* __restrict__ V,
-                                                     half_t* __restrict__ M, uint32_t Kp,
-                                                     uint64_t Pp, uint32_t C8p, uint32_t ntiles,
-                                                     uint32_t gx, uint32_t gy) {
-  static_assert(NS % 4 == 0, "step count: a multiple of 4");
+                                                        const half8* __restrict__ V,
+                                                        half_t* __restrict__ M, uint32_t Kp,
+                                                        uint64_t Pp, uint32_t C8p, uint32_t ntiles,
+                                                        uint32_t gx, uint32_t gy) {
   constexpr int TK = 4, TP = 4, BK = 256, BP = 64, CS = BP + 8;
   __shared__ __attribute__((aligned(16))) half8 Bs[2 * 4 * BP];    // B ring, 8 KB
   __shared__ __attribute__((aligned(16))) half_t Cs[BK * CS];      // epilogue tile, 36 KB
@@ -540,37 +539,45 @@ __global__ __launch_bounds__(256, 3) void k_gemm_rows_p(const half8* __restrict_
   Bs[tid] = rb[0];
   lds_barrier();
 
+  // step s (0..3) of a group of 4; `last`: the tile's last group, where
+  // A(s+2) of s = 2, 3 and B(s+4) of every s are the next tile's
+  auto step = [&](const int s, const bool last, const bool more, const Tile& nxt) {
+    const int par = s & 1;
+    half8 b[TP];
+#pragma unroll
+    for (int j = 0; j < TP; ++j) b[j] = Bs[(par * 4 + g) * BP + j * 16 + r16];
+    // transposed product (A operand = V, B operand = U): lane ends with 4
+    // consecutive tiles of one output channel
+#pragma unroll
+    for (int q = 0; q < TK; ++q)
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+        acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[j], ra[s & 1][q], acc[q][j], 0, 0, 0);
+    Bs[(par ^ 1) * 4 * BP + tid] = rb[(s + 1) & 3];  // B(s+1)
+    if (last && s == 2) {  // A(s+2) is the next tile's first step
+      pa = more ? nxt.ua : pa - sa;
+      ia = more ? sa : 0;
+    }
+    load_a(pa, ra[s & 1]);
+    pa += ia;
+    if (last && s == 0) {  // B(s+4) is the next tile's first step
+      pv = more ? nxt.vb : pv - sv;
+      iv = more ? sv : 0;
+    }
+    rb[s & 3] = *pv;
+    pv += iv;
+    lds_barrier();
+  };
+  const uint32_t ngroups = C8p / 16;  // groups of 4 steps per tile
   for (uint32_t i = 0; i < mine; ++i) {
     const bool more = i + 1 < mine;
     const Tile nxt = more ? tile(i + 1) : cur;
+    for (uint32_t grp = 1; grp < ngroups; ++grp) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int par = s & 1;
-      half8 b[TP];
-#pragma unroll
-      for (int j = 0; j < TP; ++j) b[j] = Bs[(par * 4 + g) * BP + j * 16 + r16];
-      // transposed product (A operand = V, B operand = U): lane ends with 4
-      // consecutive tiles of one output channel
-#pragma unroll
-      for (int q = 0; q < TK; ++q)
-#pragma unroll
-        for (int j = 0; j < TP; ++j)
-          acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[j], ra[s & 1][q], acc[q][j], 0, 0, 0);
-      Bs[(par ^ 1) * 4 * BP + tid] = rb[(s + 1) & 3];  // B(s+1)
-      if (s + 2 == NS) {  // A(s+2) is the next tile's first step
-        pa = more ? nxt.ua : pa - sa;
-        ia = more ? sa : 0;
-      }
-      load_a(pa, ra[s & 1]);
-      pa += ia;
-      if (s + 4 == NS) {  // B(s+4) is the next tile's first step
-        pv = more ? nxt.vb : pv - sv;
-        iv = more ? sv : 0;
-      }
-      rb[s & 3] = *pv;
-      pv += iv;
-      lds_barrier();
+      for (int s = 0; s < 4; ++s) step(s, false, more, nxt);
     }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) step(s, true, more, nxt);
     // epilogue: acc[q][j][e] is channel kb*256 + 64 wave + 16 q + r16,
     // tile pb*64 + 16 j + 4 g + e
 #pragma unroll
@@ -636,7 +643,7 @@ GemmCfg gemm_cfg(int idx) {
 }
 
 // The 256 x 64 tile as a persistent kernel (k_gemm_rows_p) when the step
-// count has an instance; false otherwise.
+// count is a multiple of 4; false otherwise.
 static bool run_rows_persistent(const Plan& pl, const void* U, const void* V, half_t* M,
                                 hipStream_t st) {
   const uint32_t ns = pl.Cp / 32, gx = (uint32_t)(pl.Pp / 64), gy = pl.Kp / 256;
@@ -653,16 +660,10 @@ static bool run_rows_persistent(const Plan& pl, const void* U, const void* V, ha
   const uint32_t need = (ntiles + 7u) & ~7u;
   if (grid > need) grid = need;
   if (grid < 8u) grid = 8u;
-#define WINO_ROWS_P(NSV)                                                                    \
-  hipLaunchKernelGGL(k_gemm_rows_p<NSV>, dim3(grid), dim3(256), 0, st, (const half8*)U,      \
-                     (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8, ntiles, gx, gy)
-  switch (ns) {
-    case 4: WINO_ROWS_P(4); return true;
-    case 8: WINO_ROWS_P(8); return true;
-    case 16: WINO_ROWS_P(16); return true;
-    default: return false;
-  }
-#undef WINO_ROWS_P
+  if (ns < 4 || ns % 4 != 0) return false;
+  hipLaunchKernelGGL(k_gemm_rows_p, dim3(grid), dim3(256), 0, st, (const half8*)U,
+                     (const half8*)V, M, pl.Kp, pl.Pp, pl.Cp / 8, ntiles, gx, gy);
+  return true;
 }
 
 template <int WK, int WP, int TK, int TP>
