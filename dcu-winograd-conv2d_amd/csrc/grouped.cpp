@@ -827,6 +827,18 @@ static size_t build_descriptors(wino_batch* b) {
     probs[j] = g;
   }
   hdr->nprob = (uint32_t)b->entries.size();
+  // LDS-staged fp16 tile kernels whose items all have an even step count
+  // >= 2 and no reduction split run their items chained
+  if (b->eng->precision == WINO_PREC_FP16) {
+    bool ok[kGemmCfgs];
+    for (int c = 0; c < kGemmCfgs; ++c) ok[c] = c < 5 && hdr->cfg_items[c] > 0;
+    for (const auto& en : b->entries) {
+      const uint32_t ns = en.gp.nsteps;
+      if (en.bp.split != 1 || ns < 2 || ns % 2) ok[en.bp.pl.gemm_cfg] = false;
+    }
+    for (int c = 0; c < kGemmCfgs; ++c)
+      if (ok[c]) hdr->chain |= 1u << c;
+  }
   for (int q = 0; q < 2; ++q) hdr->in_blocks[q] = in_first[q];
   for (int q = 0; q < 3; ++q) hdr->out_blocks[q] = out_first[q];
   hdr->nitems = (uint32_t)item_first;  // < 2^31: checked as problems are added

@@ -35,6 +35,8 @@ struct TileShape {
   static constexpr int lds_f16() {
     return 2 * 4 * (BK + BP) * 16 > BK * CS * 2 ? 2 * 4 * (BK + BP) * 16 : BK * CS * 2;
   }
+  // operand ring plus a separate epilogue tile (chained grouped items)
+  static constexpr int lds_f16_chain() { return 2 * 4 * (BK + BP) * 16 + BK * CS * 2; }
   static constexpr int lds_f8(int planes) {
     return 2 * planes * 4 * (BK + BP) * 32 > BK * CS * 2 ? 2 * planes * 4 * (BK + BP) * 32
                                                          : BK * CS * 2;
@@ -746,6 +748,173 @@ __device__ inline GWork decode_item(const GProb& pr, uint32_t i) {
   return w;
 }
 
+// A block's items of one LDS-staged fp16 tile configuration, chained: the
+// operand pipeline of gemm_tile_f16 (buffer loads two steps ahead in two
+// register sets, two LDS buffers) runs over the concatenated steps of the
+// block's items, so the last step pair of an item loads the next item's
+// first two steps, and those loads are in flight during the item's
+// epilogue; step barriers fence LDS only, so global loads stay in flight
+// across them.  Requires every item to have an even step count >= 2 and no
+// reduction split (the host sets GroupHeader::chain).  The epilogue tile has
+// its own LDS.  Accumulation order per output is that of gemm_tile_f16:
+// results are bitwise the same.
+template <int WK, int WP, int TK, int TP>
+__device__ __attribute__((always_inline)) inline void gemm_grouped_chain_f16(
+    char* smem, const GProb* __restrict__ probs, uint32_t np, uint32_t first, uint32_t end) {
+  using S = TileShape<WK, WP, TK, TP>;
+  constexpr int NT = S::NT, BK = S::BK, BP = S::BP, CS = S::CS;
+  constexpr int ALD = (4 * BK + NT - 1) / NT, BLD = (4 * BP + NT - 1) / NT;
+  half8* As = (half8*)smem;                          // [2][4][BK]
+  half8* Bs = (half8*)smem + 2 * 4 * BK;             // [2][4][BP]
+  half_t* Cs = (half_t*)((half8*)smem + 2 * 4 * (BK + BP));  // [BK][CS]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wk = wave % WK, wp = wave / WK;
+  const int g = lane >> 4, r16 = lane & 15;
+  uint32_t i = first + blockIdx.x;
+  if (i >= end) return;  // block-uniform
+  ProbCursor<ItemFirst> cur;
+  cur.init(probs, np);
+
+  struct Item {
+    __amdgpu_buffer_rsrc_t ra, rb;  // operand planes, ranges end at step s1
+    uint32_t oa[ALD], ob[BLD];      // this thread's byte offsets of step 0
+    uint32_t sa, sb;                // bytes per step
+    int s0, s1;
+    half_t* mt;                     // the item's M tile
+    uint64_t Pp;
+  };
+  auto make = [&](uint32_t ii) {
+    cur.seek(probs, np, ii);
+    const GProb& pr = probs[cur.p];
+    const GWork w = decode_item(pr, ii);
+    const uint32_t Kp = pr.Kp, C8p = pr.Cp / 8;
+    const uint64_t Pp = pr.Pp;
+    const uint32_t k0 = w.kb * BK;
+    const uint64_t p0 = w.pb * BP;
+    Item it;
+    it.ra = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const half8*)pr.U + (uint64_t)w.t * C8p * Kp + k0), 0,
+        (int)(uint32_t)(64u * w.s1 * Kp), 0x00020000);
+    it.rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const half8*)pr.V + (uint64_t)w.t * C8p * Pp + p0), 0,
+        (int)(uint32_t)(64u * w.s1 * (uint32_t)Pp), 0x00020000);
+#pragma unroll
+    for (int q = 0; q < ALD; ++q) {
+      const int e = tid + q * NT;
+      it.oa[q] = 16u * ((uint32_t)(e / BK) * Kp + (uint32_t)(e % BK));
+    }
+#pragma unroll
+    for (int q = 0; q < BLD; ++q) {
+      const int e = tid + q * NT;
+      it.ob[q] = 16u * ((uint32_t)(e / BP) * (uint32_t)Pp + (uint32_t)(e % BP));
+    }
+    it.sa = 64u * Kp;
+    it.sb = 64u * (uint32_t)Pp;
+    it.s0 = (int)w.s0;
+    it.s1 = (int)w.s1;
+    it.mt = pr.M + ((uint64_t)w.t * Kp + k0) * Pp + p0;
+    it.Pp = Pp;
+    return it;
+  };
+
+  typedef half8 aset_t[ALD];
+  typedef half8 bset_t[BLD];
+  half8 ra0[ALD], rb0[BLD], ra1[ALD], rb1[BLD];
+  auto gload = [&](const Item& it, int s, aset_t& ra, bset_t& rb) {
+#pragma unroll
+    for (int q = 0; q < ALD; ++q)
+      ra[q] = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(
+                                            it.ra, it.oa[q] + (uint32_t)s * it.sa, 0, 0));
+#pragma unroll
+    for (int q = 0; q < BLD; ++q)
+      rb[q] = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(
+                                            it.rb, it.ob[q] + (uint32_t)s * it.sb, 0, 0));
+  };
+  auto sstore = [&](int buf, const aset_t& ra, const bset_t& rb) {
+#pragma unroll
+    for (int q = 0; q < ALD; ++q) {
+      const int e = tid + q * NT;
+      if (e < 4 * BK) As[buf * 4 * BK + e] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < BLD; ++q) {
+      const int e = tid + q * NT;
+      if (e < 4 * BP) Bs[buf * 4 * BP + e] = rb[q];
+    }
+  };
+  floatx4 acc[TK][TP];
+#pragma unroll
+  for (int a = 0; a < TK; ++a)
+#pragma unroll
+    for (int c = 0; c < TP; ++c) acc[a][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto compute = [&](int buf) {
+    half8 a[TK], b[TP];
+#pragma unroll
+    for (int q = 0; q < TK; ++q) a[q] = As[(buf * 4 + g) * BK + wk * TK * 16 + q * 16 + r16];
+#pragma unroll
+    for (int c = 0; c < TP; ++c) b[c] = Bs[(buf * 4 + g) * BP + wp * TP * 16 + c * 16 + r16];
+#pragma unroll
+    for (int q = 0; q < TK; ++q)
+#pragma unroll
+      for (int c = 0; c < TP; ++c)
+        acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[q], b[c], acc[q][c], 0, 0, 0);
+  };
+
+  // steps s0 + 2j of an item sit in set 0 / LDS buffer 0, s0 + 2j + 1 in set 1
+  Item it = make(i);
+  gload(it, it.s0, ra0, rb0);
+  gload(it, it.s0 + 1, ra1, rb1);
+  sstore(0, ra0, rb0);
+  lds_barrier();
+  for (;;) {
+    const uint32_t inext = i + gridDim.x;
+    const bool more = inext < end;
+    // the next item, or this one past its last step (zeros) after the last
+    const Item nx = more ? make(inext) : it;
+    const int ns0 = more ? nx.s0 : it.s1;
+    for (int s = it.s0; s + 2 < it.s1; s += 2) {
+      gload(it, s + 2, ra0, rb0);
+      compute(0);
+      sstore(1, ra1, rb1);
+      lds_barrier();
+      gload(it, s + 3, ra1, rb1);
+      compute(1);
+      sstore(0, ra0, rb0);
+      lds_barrier();
+    }
+    // last step pair: load the next item's first two steps
+    gload(nx, ns0, ra0, rb0);
+    compute(0);
+    sstore(1, ra1, rb1);
+    lds_barrier();
+    gload(nx, ns0 + 1, ra1, rb1);
+    compute(1);
+    sstore(0, ra0, rb0);
+    // epilogue: acc -> fp16 tile in its own LDS -> 16-byte row stores of M
+#pragma unroll
+    for (int q = 0; q < TK; ++q)
+#pragma unroll
+      for (int c = 0; c < TP; ++c) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = wk * TK * 16 + q * 16 + 4 * g + e;
+          const int col = wp * TP * 16 + c * 16 + r16;
+          Cs[row * CS + col] = (half_t)acc[q][c][e];
+        }
+        acc[q][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+    lds_barrier();  // also orders the last sstore before the next item's first step
+    constexpr int CHUNKS = BK * BP / 8;
+    for (int q = tid; q < CHUNKS; q += NT) {
+      const int row = q / (BP / 8), c8 = q % (BP / 8);
+      *(half8*)(it.mt + (uint64_t)row * it.Pp + c8 * 8) = *(const half8*)(Cs + row * CS + c8 * 8);
+    }
+    if (!more) break;
+    it = nx;
+    i = inext;
+  }
+}
+
 // One launch per tile configuration: each kernel walks only the items of its
 // configuration, so its register and LDS budget is that tile's alone.
 template <int CFG>
@@ -754,6 +923,13 @@ __global__ __launch_bounds__(256) void k_gemm_grouped_f16(const GroupHeader* __r
   extern __shared__ __attribute__((aligned(16))) char gsmem[];
   const uint32_t first = hdr->cfg_first[CFG], end = first + hdr->cfg_items[CFG];
   const uint32_t np = hdr->nprob;
+  if constexpr (CFG < 5) {
+    if ((hdr->chain >> CFG) & 1u) {
+      gemm_grouped_chain_f16<kCfgT<CFG>::WK, kCfgT<CFG>::WP, kCfgT<CFG>::TK, kCfgT<CFG>::TP>(
+          gsmem, probs, np, first, end);
+      return;
+    }
+  }
   ProbCursor<ItemFirst> cur;
   cur.init(probs, np);
   for (uint32_t i = first + blockIdx.x; i < end; i += gridDim.x) {
@@ -805,7 +981,8 @@ template <int CFG>
 static size_t cfg_lds(int precision) {
   using C = kCfgT<CFG>;
   using S = TileShape<C::WK, C::WP, C::TK, C::TP>;
-  if (precision == WINO_PREC_FP16) return (size_t)S::lds_f16();
+  if (precision == WINO_PREC_FP16)
+    return CFG < 5 ? (size_t)S::lds_f16_chain() : (size_t)S::lds_f16();
   return (size_t)S::lds_f8(precision == WINO_PREC_FP8X2 ? 2 : 1);
 }
 

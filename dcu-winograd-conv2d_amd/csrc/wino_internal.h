@@ -111,7 +111,9 @@ struct GroupHeader {
   uint32_t nprob;
   uint32_t nitems;        // GEMM work items (tiles x reduction splits, all problems)
   uint32_t amax_blocks;   // total virtual blocks of the fp8 input reduction
-  uint32_t pad;
+  uint32_t chain;         // bit c: every item of fp16 tile configuration c (< 5)
+                          // has an even step count >= 2 and no split, so its
+                          // grouped GEMM chains the items (see kernels_gemm)
   uint32_t in_blocks[2];  // input-transform virtual blocks: [0] F(2x3), [1] F(4x3) problems
   uint32_t out_blocks[3]; // output: [0] F(2x3), [1] F(4x3) LDS-staged blocks, [2] split pairs
   uint32_t pad2;

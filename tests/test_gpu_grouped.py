@@ -110,6 +110,48 @@ def test_batch_matches_single_problem_and_fp64(wc, gpu, oracle):
         S.close()
 
 
+def test_chained_items_bitwise(wc, gpu):
+    """The LDS-staged tile kernels chain a block's items (the next item's
+    first loads issued during the current one's last steps) when every item
+    of the tile has an even step count and no split.  A batch with more items
+    than the 1024 blocks of the grouped GEMM (N=48 C=128 28x28 K=256, 2 or 4
+    items per block, and N=6 C=40 20x20 K=256), forced onto each of those
+    tiles without split, equals the single-problem operator bitwise."""
+    import torch
+    eng = wc.Engine(FP16, 1 << 30)
+    shapes = [(48, 128, 28, 28, 256), (6, 40, 20, 20, 256)]  # 4 and 2 steps of 32 channels
+    data, fids, ref = [], [], []
+    try:
+        for q, (n, c, h, w, k) in enumerate(shapes):
+            x, wt = rng_inputs(500 + q, n, c, h, w, k, dist="clip4")
+            dx, dw = gpu.to_dev(x), gpu.to_dev(wt)
+            data.append((dx, dw))
+            fids.append(eng.register_filter(dw.data_ptr(), k, c, F43))
+            ref.append(gpu.run_conv(wc, x, wt, algo=F43)[0])
+        torch.cuda.synchronize()
+        for cfg in range(5):
+            b = wc.Batch(eng, 4)
+            try:
+                ys = []
+                for q, (n, c, h, w, k) in enumerate(shapes):
+                    y = gpu.empty_f16((n, k, h, w))
+                    ys.append(y)
+                    idx = b.add(fids[q], data[q][0].data_ptr(), y.data_ptr(), n, h, w)
+                    b.set_tile(idx, cfg, 1)
+                torch.cuda.synchronize()
+                items = [b.info(q).tiles for q in range(len(shapes))]
+                say(f"chained tiles cfg={cfg} tiles={items}")
+                b.launch()
+                torch.cuda.synchronize()
+                for q in range(len(shapes)):
+                    got = gpu.to_host(ys[q])
+                    assert np.array_equal(got.view(np.uint16), ref[q].view(np.uint16)), (cfg, q)
+            finally:
+                b.destroy()
+    finally:
+        eng.destroy()
+
+
 @pytest.mark.parametrize("j", [7, 9, 10])
 def test_tile_shapes_bitwise_equal(wc, gpu, j):
     """Every tile shape the filter's K padding allows; 256-row tiles only
