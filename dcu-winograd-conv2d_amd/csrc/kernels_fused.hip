@@ -12,7 +12,6 @@
 //    loaded two 32-channel steps ahead into registers (two alternating sets,
 //    branch-free loads with per-lane offsets computed once) and written to
 //    LDS at the top of its step;
-//  * the input transform runs in packed fp16 on channel pairs;
 //  * 8 waves: wave w owns tile-position row xi = w % 4 (t = 4 xi .. 4 xi + 3)
 //    for output channels (w / 4) * 32 .. +31, so the column pass of A^T M A
 //    runs in registers and only its 2 partial rows go through LDS;
@@ -44,29 +43,29 @@ struct FusedGeom {
   uint32_t CU;           // load units per row (dwords if even W, else halves)
   uint32_t units;        // load units per channel (NB * LH * CU)
   uint64_t CHW;          // C * H * W
+  float one;             // 1.0f (see input2d_f23_mix)
 };
 
-// v = B^T d B of F(2x3) for two channels at once, in packed fp16 arithmetic
-// (B^T has entries 0 and +-1: 32 packed adds, no conversions).  Same order
-// of operations as input2d<2>: columns first, then rows; each sum is
-// rounded to fp16, as the reference's fp16 transform rounds it
-// (src/winograd_2x3_non_fused.cpp:324-377).
-__device__ inline void input2d_f23_h2(const half2_t* d, half2_t* v) {
-  half2_t t[16];
+// v = B^T d B of F(2x3) from fp16 inputs, in FP32: the column pass is
+// fma(d_i, 1, +-d_j) on the fp16 values (v_fma_mix_f32 reads them straight
+// from their packed registers, no separate conversion; one FP32 rounding,
+// the same value as converting first and subtracting), the row pass plain
+// FP32 adds.  Same result as input2d<2> on the converted inputs.
+// `one` is 1.0 from a kernel argument: a literal 1 would let the compiler
+// turn fma(x, 1, y) back into a conversion and an add.
+__device__ inline void input2d_f23_mix(const half_t* d, float one, float* v) {
+  float t[16];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    t[0 * 4 + j] = d[0 * 4 + j] - d[2 * 4 + j];
-    t[1 * 4 + j] = d[1 * 4 + j] + d[2 * 4 + j];
-    t[2 * 4 + j] = d[2 * 4 + j] - d[1 * 4 + j];
-    t[3 * 4 + j] = d[1 * 4 + j] - d[3 * 4 + j];
+    const float d0 = (float)d[0 * 4 + j], d1 = (float)d[1 * 4 + j], d2 = (float)d[2 * 4 + j],
+                d3 = (float)d[3 * 4 + j];
+    t[0 * 4 + j] = __builtin_fmaf(d0, one, -d2);
+    t[1 * 4 + j] = __builtin_fmaf(d1, one, d2);
+    t[2 * 4 + j] = __builtin_fmaf(d2, one, -d1);
+    t[3 * 4 + j] = __builtin_fmaf(d1, one, -d3);
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    v[a * 4 + 0] = t[a * 4 + 0] - t[a * 4 + 2];
-    v[a * 4 + 1] = t[a * 4 + 1] + t[a * 4 + 2];
-    v[a * 4 + 2] = t[a * 4 + 2] - t[a * 4 + 1];
-    v[a * 4 + 3] = t[a * 4 + 1] - t[a * 4 + 3];
-  }
+  for (int a = 0; a < 4; ++a) Tf<2>::BT(t + 4 * a, v + 4 * a);
 }
 
 // EVENW: W even, so every staged row starts on a dword and is loaded as
@@ -192,34 +191,34 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
   const int lw2 = (int)g.LWh >> 1;
 
   auto transform = [&](half8* Vout) {
-    // d[pixel] = (channel 2cp, channel 2cp + 1) as one half2
-    const uint32_t* s0 = (const uint32_t*)(xs + (2 * cp) * CH_H) + src0;
-    const uint32_t* s1 = (const uint32_t*)(xs + (2 * cp + 1) * CH_H) + src0;
-    half2_t d[16], vo[16];
+    half2_t vo[16];
+    {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const uint32_t a0 = s0[a * lw2], a1 = s0[a * lw2 + 1];
-      const uint32_t b0 = s1[a * lw2], b1 = s1[a * lw2 + 1];
-      // v_perm_b32 (src0 = channel 2cp+1, src1 = channel 2cp): selector
-      // 0x05040100 packs the two low halves, 0x07060302 the two high halves
-      constexpr uint32_t LO = 0x05040100u, HI = 0x07060302u;
-      uint32_t e[4];
-      if constexpr (EVENW) {
-        const uint32_t a2 = s0[a * lw2 + 2], b2 = s1[a * lw2 + 2];
-        e[0] = __builtin_amdgcn_perm(b0, a0, HI);
-        e[1] = __builtin_amdgcn_perm(b1, a1, LO);
-        e[2] = __builtin_amdgcn_perm(b1, a1, HI);
-        e[3] = __builtin_amdgcn_perm(b2, a2, LO);
-      } else {
-        e[0] = __builtin_amdgcn_perm(b0, a0, LO);
-        e[1] = __builtin_amdgcn_perm(b0, a0, HI);
-        e[2] = __builtin_amdgcn_perm(b1, a1, LO);
-        e[3] = __builtin_amdgcn_perm(b1, a1, HI);
+      for (int cc = 0; cc < 2; ++cc) {
+        const uint32_t* src = (const uint32_t*)(xs + (2 * cp + cc) * CH_H) + src0;
+        half_t d[16];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const uint32_t w0 = src[a * lw2], w1 = src[a * lw2 + 1];
+          if constexpr (EVENW) {
+            const uint32_t w2 = src[a * lw2 + 2];
+            d[a * 4 + 0] = __builtin_bit_cast(half_t, (uint16_t)(w0 >> 16));
+            d[a * 4 + 1] = __builtin_bit_cast(half_t, (uint16_t)w1);
+            d[a * 4 + 2] = __builtin_bit_cast(half_t, (uint16_t)(w1 >> 16));
+            d[a * 4 + 3] = __builtin_bit_cast(half_t, (uint16_t)w2);
+          } else {
+            d[a * 4 + 0] = __builtin_bit_cast(half_t, (uint16_t)w0);
+            d[a * 4 + 1] = __builtin_bit_cast(half_t, (uint16_t)(w0 >> 16));
+            d[a * 4 + 2] = __builtin_bit_cast(half_t, (uint16_t)w1);
+            d[a * 4 + 3] = __builtin_bit_cast(half_t, (uint16_t)(w1 >> 16));
+          }
+        }
+        float v[16];
+        input2d_f23_mix(d, g.one, v);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) vo[t][cc] = (half_t)v[t];
       }
-#pragma unroll
-      for (int b = 0; b < 4; ++b) d[a * 4 + b] = __builtin_bit_cast(half2_t, e[b]);
     }
-    input2d_f23_h2(d, vo);
     // V entry (t, 8-channel group cgrp, tile it), halves 2q .. 2q+1
     half2_t* V2 = (half2_t*)Vout;
 #pragma unroll
@@ -387,6 +386,7 @@ hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, ui
   g.LWh = evenw ? 2 * g.CU : g.CU;
   g.units = g.NB * g.LH * g.CU;
   g.CHW = (uint64_t)C * H * W;
+  g.one = 1.0f;
   const uint64_t regions = (uint64_t)((N + g.NB - 1) / g.NB) * g.nrh * g.nrw;
   const uint32_t blocks = (uint32_t)(regions * g.kblocks);
   const size_t stage_b = (size_t)2 * 16 * 4 * kFusedPB * 16 + (size_t)32 * 400 * 2;
