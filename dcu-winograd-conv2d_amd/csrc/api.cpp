@@ -121,9 +121,9 @@ int make_plan(const problem_t* pr, int algo, int precision, Plan* pl) {
   if (!winograd_ok(pr)) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F43_FUSED) return WINO_ERR_UNSUPPORTED;
   if (algo == WINO_ALGO_F23_FUSED && precision != WINO_PREC_FP16) return WINO_ERR_UNSUPPORTED;
-  // The fused kernel's input offsets are 32-bit below a sentinel: input of
-  // at most 1 GiB.  Larger problems the reference selector gives to it run
-  // the non-fused F(2x3) under AUTO.
+  // The fused kernel's input and U offsets are 32-bit: input of at most
+  // 1 GiB, U below 2 GiB.  Larger problems the reference selector gives to
+  // it run the non-fused F(2x3) under AUTO.
   if (algo == WINO_ALGO_F23_FUSED && !fused_input_fits(pr)) {
     if (!auto_algo) return WINO_ERR_UNSUPPORTED;
     algo = WINO_ALGO_F23;

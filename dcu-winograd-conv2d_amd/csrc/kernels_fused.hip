@@ -43,6 +43,8 @@ struct FusedGeom {
   uint32_t CU;           // load units per row (dwords if even W, else halves)
   uint32_t units;        // load units per channel (NB * LH * CU)
   uint64_t CHW;          // C * H * W
+  uint64_t HW;           // H * W
+  uint32_t ubytes;       // bytes of U (16 x Cp x Kp fp16, < 2^31)
   float one;             // 1.0f (see input2d_f23_mix)
 };
 
@@ -132,11 +134,13 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
   auto prefetch = [&](int s) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+      // branch-free (scalar selects): a channel >= C gets base X, range 0
       const uint32_t c = 32u * s + 4u * wave + i;
-      const uint64_t cb = 2ull * c * g.H * g.W;
-      const uint32_t range = c < g.C ? (uint32_t)(total_b - cb) : 0u;
+      const bool in = c < g.C;
+      const uint64_t cb = 2ull * (in ? c : 0u) * g.HW;
+      const uint32_t range = in ? (uint32_t)(total_b - cb) : 0u;
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)((const char*)X + (c < g.C ? cb : 0)), 0, (int)range, 0x00020000);
+          (void*)((const char*)X + cb), 0, (int)range, 0x00020000);
 #pragma unroll
       for (int j = 0; j < MAXP; ++j) {
         if constexpr (EVENW)
@@ -165,13 +169,23 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_f23(const half_t* __res
   // U fragments of this wave for one step: t = 4 xi + tl, k rows kh*32 + kt*16 + r16
   typedef half8 ufrag_t[4][2];
   half8 ua[4][2], ub[4][2];
+  // U through one buffer resource: a per-lane byte offset (8-channel
+  // group, output channel), the tile position and step in the wave-uniform
+  // soffset, the 16-row half as an immediate.  U is below 2 GiB (checked by
+  // the planner).
+  const __amdgpu_buffer_rsrc_t urs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)U, 0, (int)g.ubytes, 0x00020000);
+  const uint32_t uoff = ((uint32_t)grp * g.Kp + kb * KB + kh * 32 + r16) * 16u;
   auto uload = [&](int s, ufrag_t& dst) {
 #pragma unroll
-    for (int tl = 0; tl < 4; ++tl)
+    for (int tl = 0; tl < 4; ++tl) {
+      const uint32_t so = __builtin_amdgcn_readfirstlane(
+          (((uint32_t)(4 * xi + tl) * g.C8p + 4u * (uint32_t)s) * g.Kp) * 16u);
 #pragma unroll
       for (int kt = 0; kt < 2; ++kt)
-        dst[tl][kt] = U[((uint64_t)(4 * xi + tl) * g.C8p + 4 * s + grp) * g.Kp + kb * KB +
-                        kh * 32 + kt * 16 + r16];
+        dst[tl][kt] = __builtin_bit_cast(
+            half8, __builtin_amdgcn_raw_buffer_load_b128(urs, uoff + kt * 256u, so, 0));
+    }
   };
 
   // transform phase: thread -> (tile it, channel pair cp = 4 * cgrp + q);
@@ -373,8 +387,9 @@ hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, ui
   g.OH = pl.oh;
   g.OW = pl.ow;
   const bool evenw = (W % 2) == 0;
-  // buffer byte offsets must stay below the out-of-range sentinel
-  if (2ull * N * C * H * W > (1ull << 30)) return hipErrorInvalidValue;
+  // 32-bit buffer offsets: input below the out-of-range sentinel, U below 2 GiB
+  if (2ull * N * C * H * W > (1ull << 30) || 32ull * pl.Cp * pl.Kp >= (1ull << 31))
+    return hipErrorInvalidValue;
   if (!fused_region(N, g.TH, g.TW, evenw, g.RH, g.RW, g.NB)) return hipErrorInvalidValue;
   g.nrh = (g.TH + g.RH - 1) / g.RH;
   g.nrw = (g.TW + g.RW - 1) / g.RW;
@@ -386,6 +401,8 @@ hipError_t launch_fused_f23(const Plan& pl, const _Float16* x, const void* U, ui
   g.LWh = evenw ? 2 * g.CU : g.CU;
   g.units = g.NB * g.LH * g.CU;
   g.CHW = (uint64_t)C * H * W;
+  g.HW = (uint64_t)H * W;
+  g.ubytes = (uint32_t)(16ull * g.C8p * g.Kp * 16);
   g.one = 1.0f;
   const uint64_t regions = (uint64_t)((N + g.NB - 1) / g.NB) * g.nrh * g.nrw;
   const uint32_t blocks = (uint32_t)(regions * g.kblocks);

@@ -62,10 +62,13 @@ int reference_select(const problem_t* pr);
 int validate(const problem_t* pr);
 bool winograd_ok(const problem_t* pr);  // 3x3, stride 1, pad 1
 int choose_gemm_cfg(uint32_t K, uint32_t C, uint64_t P, int t2, int precision, uint32_t kmax);
-// Input of the fused F(2x3) kernel: at most 1 GiB (its 32-bit load offsets
-// stay below an out-of-range sentinel).
+// The fused F(2x3) kernel addresses its input and the transformed filter U
+// (16 x Cp x Kp fp16, Cp = C rounded up to 32, Kp = K rounded up to 64)
+// with 32-bit buffer offsets: input at most 1 GiB (offsets stay below an
+// out-of-range sentinel), U below 2 GiB.
 inline bool fused_input_fits(const problem_t* pr) {
-  return 2ull * pr->n * pr->c * pr->h * pr->w <= (1ull << 30);
+  const uint64_t cp = (pr->c + 31ull) / 32ull * 32ull, kp = (pr->k + 63ull) / 64ull * 64ull;
+  return 2ull * pr->n * pr->c * pr->h * pr->w <= (1ull << 30) && 32ull * cp * kp < (1ull << 31);
 }
 // One tile position's plane of U (Cp x Kp) and of V (Cp x Pp), fp16, is
 // below 4 GiB: the LDS-staged GEMM tiles address a plane through buffer

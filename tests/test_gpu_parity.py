@@ -268,6 +268,19 @@ def test_auto_fused_input_limit(wc, gpu):
         op.destroy()
 
 
+def test_fused_filter_limit(wc, gpu):
+    """The fused kernel reads the transformed filter U (16 x Cp x Kp fp16)
+    through 32-bit buffer offsets, so U must stay below 2 GiB: C = K = 8192
+    (U exactly 2 GiB) is refused when the fused algorithm is forced.  Under
+    AUTO the 1 GiB input limit and the selector's img >= 10 flt rule keep
+    C*K far below that.  Planning only (refused before any allocation)."""
+    n, c, h, k = 1, 8192, 8, 8192
+    assert 32 * c * k == 1 << 31
+    with pytest.raises(wc.WinoError) as e:
+        wc.Conv2d(0, 0, 0, n, c, h, h, k, algo=F23F)
+    assert e.value.status == wc.ERR_UNSUPPORTED
+
+
 @pytest.mark.parametrize("algo,prec", [(F43, FP16), (F23F, FP16), (F43, FP8X2), (F23, FP16)])
 def test_n64_c256_28x28_k256_sampled(wc, gpu, oracle, algo, prec):
     """N=64 C=256 28x28 K=256 (the throughput configuration), 1% sample."""
