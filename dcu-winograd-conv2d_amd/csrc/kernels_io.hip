@@ -466,7 +466,26 @@ static size_t input_lds(const Plan& pl, const InputGeom& g) {
 hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N, uint32_t C,
                                   uint32_t H, uint32_t W, void* V, const float* amax,
                                   float* v_scale, hipStream_t st) {
-  const InputGeom g = input_geom(pl, N, C, H, W, false, 40u * 1024u);
+  InputGeom g = input_geom(pl, N, C, H, W, false, 40u * 1024u);
+  // A small problem (few images, or an LDS-bound window) gets few blocks:
+  // halve the window's tile rows while the launch has under 4 blocks per CU
+  // and the window keeps at least half a transform pass of tiles (so one
+  // channel group per block stays right).
+  {
+    static int cus = 0;
+    if (cus <= 0) {
+      int dev = 0, n = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+      cus = n > 0 ? n : 256;
+    }
+    const uint32_t half_pass = 256u / (pl.precision == WINO_PREC_FP16 ? 8u : 32u);
+    while (g.G == 1 && g.WR > 1 && (g.WR / 2) * g.WC >= half_pass &&
+           (uint64_t)N * g.nwr * g.nwc * g.NGB < 4ull * (uint64_t)cus) {
+      g.WR /= 2;
+      g.nwr = div_up(g.TH, g.WR);
+    }
+  }
   const size_t lds = input_lds(pl, g);
   dim3 grid((uint32_t)((uint64_t)N * g.nwr * g.nwc), g.NGB);
 #define WINO_IN(M, G, P)                                                                      \
