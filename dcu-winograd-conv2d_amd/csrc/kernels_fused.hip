@@ -5,13 +5,15 @@
 // GEMMs and the output transform in one launch, transformed tiles never
 // leaving LDS/registers.  Re-designed for CDNA4:
 //  * the filter is pre-transformed once (resident U, fp16 [16][C/8][Kp][8]);
-//    MFMA A-fragments are read straight from U (one 16-byte load per lane),
-//    one 32-channel step ahead;
+//    MFMA A-fragments are read straight from U (one 16-byte buffer load per
+//    lane, no per-load address arithmetic), one 32-channel step ahead;
 //  * a block owns a spatial region (NB images x RH tile rows x RW tile
 //    columns, <= 32 tiles) and 64 output channels.  The region's input is
 //    loaded two 32-channel steps ahead into registers (two alternating sets,
 //    branch-free loads with per-lane offsets computed once) and written to
 //    LDS at the top of its step;
+//  * the input transform is FP32; its column pass reads the fp16 inputs
+//    straight into v_fma_mix_f32 (input2d_f23_mix);
 //  * 8 waves: wave w owns tile-position row xi = w % 4 (t = 4 xi .. 4 xi + 3)
 //    for output channels (w / 4) * 32 .. +31, so the column pass of A^T M A
 //    runs in registers and only its 2 partial rows go through LDS;
