@@ -463,27 +463,70 @@ static size_t input_lds(const Plan& pl, const InputGeom& g) {
 
 // ------------------------------------------------------ single problem
 
+// The single-problem input kernel for (m, precision, even width).
+static const void* input_kernel(int m, int precision, bool evenw) {
+  if (precision == WINO_PREC_FP16)
+    return m == 2 ? (evenw ? (const void*)k_input<2, 8, 0, true> : (const void*)k_input<2, 8, 0, false>)
+                  : (evenw ? (const void*)k_input<4, 8, 0, true> : (const void*)k_input<4, 8, 0, false>);
+  if (precision == WINO_PREC_FP8)
+    return m == 2 ? (evenw ? (const void*)k_input<2, 32, 1, true> : (const void*)k_input<2, 32, 1, false>)
+                  : (evenw ? (const void*)k_input<4, 32, 1, true> : (const void*)k_input<4, 32, 1, false>);
+  return m == 2 ? (evenw ? (const void*)k_input<2, 32, 2, true> : (const void*)k_input<2, 32, 2, false>)
+                : (evenw ? (const void*)k_input<4, 32, 2, true> : (const void*)k_input<4, 32, 2, false>);
+}
+
+// Resident 256-thread blocks per CU of an input kernel at `lds` bytes of
+// dynamic LDS (0 if the query fails), remembered per thread.
+static int input_occupancy(const void* kfn, size_t lds) {
+  struct Entry {
+    const void* f;
+    size_t lds;
+    int blocks;
+  };
+  thread_local Entry cache[16];
+  thread_local int used = 0;
+  for (int i = 0; i < used; ++i)
+    if (cache[i].f == kfn && cache[i].lds == lds) return cache[i].blocks;
+  int r = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, kfn, 256, lds) != hipSuccess) r = 0;
+  if (used < 16) cache[used++] = Entry{kfn, lds, r};
+  return r;
+}
+
 hipError_t launch_input_transform(const Plan& pl, const _Float16* x, uint32_t N, uint32_t C,
                                   uint32_t H, uint32_t W, void* V, const float* amax,
                                   float* v_scale, hipStream_t st) {
   InputGeom g = input_geom(pl, N, C, H, W, false, 40u * 1024u);
+  static int cus = 0;
+  if (cus <= 0) {
+    int dev = 0, n = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    cus = n > 0 ? n : 256;
+  }
+  const uint32_t CG = pl.precision == WINO_PREC_FP16 ? 8u : 32u;
   // A small problem (few images, or an LDS-bound window) gets few blocks:
   // halve the window's tile rows while the launch has under 4 blocks per CU
   // and the window keeps at least half a transform pass of tiles (so one
   // channel group per block stays right).
-  {
-    static int cus = 0;
-    if (cus <= 0) {
-      int dev = 0, n = 0;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
-      cus = n > 0 ? n : 256;
-    }
-    const uint32_t half_pass = 256u / (pl.precision == WINO_PREC_FP16 ? 8u : 32u);
-    while (g.G == 1 && g.WR > 1 && (g.WR / 2) * g.WC >= half_pass &&
-           (uint64_t)N * g.nwr * g.nwc * g.NGB < 4ull * (uint64_t)cus) {
-      g.WR /= 2;
-      g.nwr = div_up(g.TH, g.WR);
+  while (g.G == 1 && g.WR > 1 && (g.WR / 2) * g.WC >= 256u / CG &&
+         (uint64_t)N * g.nwr * g.nwc * g.NGB < 4ull * (uint64_t)cus) {
+    g.WR /= 2;
+    g.nwr = div_up(g.TH, g.WR);
+  }
+  // A large one: when its blocks need more than one round of resident
+  // blocks but with two channel groups each they all fit in one, give each
+  // block two groups (no partly filled last round).
+  if (g.G == 1 && g.NG >= 2) {
+    InputGeom g2 = g;
+    g2.G = 2;
+    g2.NGB = div_up(g.NG, 2);
+    const size_t lds1 = input_lds(pl, g), lds2 = input_lds(pl, g2);
+    const void* kfn = input_kernel(pl.m, pl.precision, W % 2 == 0);
+    if (lds2 <= 40u * 1024u) {
+      const uint64_t r1 = (uint64_t)input_occupancy(kfn, lds1), r2 = (uint64_t)input_occupancy(kfn, lds2);
+      const uint64_t tiles_blocks = (uint64_t)N * g.nwr * g.nwc;
+      if (r2 > 0 && tiles_blocks * g.NGB > r1 * cus && tiles_blocks * g2.NGB <= r2 * cus) g = g2;
     }
   }
   const size_t lds = input_lds(pl, g);
